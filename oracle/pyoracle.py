@@ -1,0 +1,254 @@
+"""ctypes front-ends of the test-infrastructure CPU builds.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of bench.py -- never by the product package.
+
+* :class:`OracleSim` drives ``oracle/liboracle_fx.so``, the C restatement of the
+  reference RK2 stage (oracle/fx_oracle.c).
+* :class:`RefSim` drives ``oracle/_ref/libprj_ref_{hydro,mhd}.so``, the serial
+  build of the reference sources with ``oracle/ref_harness.c`` as its driver.
+
+Both take and return "global" arrays over active cells: cells ``[var][X][Y][Z]``
+(Z fastest), faces ``Bx[X+1][Y][Z]``, ``By[X][Y+1][Z]``, ``Bz[X][Y][Z+1]``.
+Boundary codes follow the reference (src/prj_defs.h:594-599): 0 outflow,
+1 reflect, 3 periodic.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle_fx.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def ensure_built():
+    """Build the restatement (and, where the reference tree exists, _ref)."""
+    if not os.path.exists(ORACLE_SO):
+        subprocess.run(["make", "-C", HERE, "oracle"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+class _Grid(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("bc", C.c_int * 6), ("dx", C.c_double * 3),
+                ("area", C.c_double * 3), ("vol", C.c_double), ("gamma", C.c_double),
+                ("mhd", C.c_int), ("eta", C.c_double)]
+
+
+_oracle_lib = None
+
+
+def oracle_lib():
+    global _oracle_lib
+    if _oracle_lib is None:
+        ensure_built()
+        lib = C.CDLL(ORACLE_SO, mode=C.RTLD_LOCAL)
+        lib.fxo_grid_init.argtypes = [C.POINTER(_Grid), C.c_int, C.c_int, C.c_int, _dp, _ip,
+                                      C.c_double, C.c_int, C.c_double]
+        lib.fxo_calc_dt.restype = C.c_double
+        lib.fxo_calc_dt.argtypes = [C.POINTER(_Grid), _dp, C.c_double]
+        lib.fxo_stage.argtypes = [C.POINTER(_Grid), C.c_int, C.c_double, _dp, _dp, _dp,
+                                  C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
+                                  C.POINTER(_dp)]
+        lib.fxo_bf2bc.argtypes = [C.POINTER(_Grid), C.POINTER(_dp), _dp]
+        lib.fxo_mc_slope.restype = C.c_double
+        lib.fxo_mc_slope.argtypes = [C.c_double] * 3
+        lib.fxo_hllc.argtypes = [_dp, _dp] + [C.c_double] * 4 + [_dp, _dp] + [C.c_double] * 3
+        lib.fxo_hlld.argtypes = [_dp, _dp] + [C.c_double] * 5 + [_dp, _dp, _dp, _dp]
+        lib.fxo_emf_upwind.argtypes = [_dp, _dp, _dp, _dp]
+        lib.fxo_last_error.restype = C.c_char_p
+        _oracle_lib = lib
+    return _oracle_lib
+
+
+def face_shapes(n):
+    nx, ny, nz = n
+    return [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+
+
+def edge_shapes(n):
+    nx, ny, nz = n
+    return [(nx, ny + 1, nz + 1), (nx + 1, ny, nz + 1), (nx + 1, ny + 1, nz)]
+
+
+class OracleSim:
+    """RK2 stepping with the C restatement on global arrays."""
+
+    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0):
+        self.lib = oracle_lib()
+        self.g = _Grid()
+        xl = np.ascontiguousarray(xlim, dtype=np.float64)
+        bcs = (C.c_int * 6)(*bc)
+        rc = self.lib.fxo_grid_init(C.byref(self.g), int(n[0]), int(n[1]), int(n[2]),
+                                    _ptr(xl), bcs, float(gamma), int(mhd), float(eta))
+        if rc != 0:
+            raise ValueError(self.lib.fxo_last_error().decode())
+        self.n = tuple(int(v) for v in n)
+        self.mhd = bool(mhd)
+        self.nh = 9 if mhd else 6
+        self.dx = np.array(self.g.dx[:])
+        self.dt = 0.0
+        self.time = 0.0
+
+    def set_state(self, W, bf=None):
+        self.W0 = np.ascontiguousarray(W, dtype=np.float64).copy()
+        self.W1 = self.W0.copy()
+        if self.mhd:
+            self.bf0 = [np.ascontiguousarray(b, dtype=np.float64).copy() for b in bf]
+            self.bf1 = [b.copy() for b in self.bf0]
+            self.bf2bc(self.W0, self.bf0)
+            self.W1[...] = self.W0
+        self.dt = 0.0
+        self.time = 0.0
+
+    def bf2bc(self, W, bf):
+        arr = (_dp * 3)(*[_ptr(b) for b in bf])
+        self.lib.fxo_bf2bc(C.byref(self.g), arr, _ptr(W))
+
+    def calc_dt(self, cfl):
+        return self.lib.fxo_calc_dt(C.byref(self.g), _ptr(self.W0), float(cfl))
+
+    def stage(self, s, dt, emf_out=None):
+        out = np.empty_like(self.W0)
+        nullb = C.cast(None, C.POINTER(_dp))
+        if self.mhd:
+            bfo = [np.empty_like(b) for b in self.bf0]
+            b0 = (_dp * 3)(*[_ptr(b) for b in self.bf0])
+            b1 = (_dp * 3)(*[_ptr(b) for b in self.bf1])
+            bo = (_dp * 3)(*[_ptr(b) for b in bfo])
+            eo = (_dp * 3)(*[_ptr(e) for e in emf_out]) if emf_out is not None else nullb
+        else:
+            b0 = b1 = bo = eo = nullb
+        rc = self.lib.fxo_stage(C.byref(self.g), int(s), float(dt), _ptr(self.W0),
+                                _ptr(self.W1), _ptr(out), b0, b1, bo, eo)
+        if rc != 0:
+            raise RuntimeError(self.lib.fxo_last_error().decode())
+        if s == 1:
+            self.W1 = out
+            if self.mhd:
+                self.bf1 = bfo
+        else:
+            self.W0 = out
+            if self.mhd:
+                self.bf0 = bfo
+
+    def step(self, cfl, dt_factor=1.2):
+        """src/main.c:510-545: dt from CFL with the growth limit, then RK2."""
+        dt_new = self.calc_dt(cfl)
+        if self.dt > 0.0:
+            lim = dt_factor * self.dt
+            self.dt = dt_new if dt_new <= lim else lim
+        else:
+            self.dt = dt_new
+        self.stage(1, self.dt)
+        self.stage(2, self.dt)
+        self.time += self.dt
+        return self.dt
+
+
+_ref_libs = {}
+
+
+def ref_available():
+    return all(os.path.exists(os.path.join(REF_DIR, f"libprj_ref_{k}.so"))
+               for k in ("hydro", "mhd"))
+
+
+def ref_lib(mhd):
+    key = "mhd" if mhd else "hydro"
+    if key not in _ref_libs:
+        path = os.path.join(REF_DIR, f"libprj_ref_{key}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        lib = C.CDLL(path, mode=C.RTLD_LOCAL)
+        lib.refh_init.argtypes = [C.c_int] * 3 + [_dp, _ip, C.c_double, C.c_double]
+        lib.refh_set_prim.argtypes = [C.c_int, _dp]
+        lib.refh_get_prim.argtypes = [C.c_int, _dp]
+        lib.refh_get_eosvar.argtypes = [_dp, _dp, _dp]
+        lib.refh_geometry.argtypes = [_dp, _dp, _dp]
+        lib.refh_calc_dt.restype = C.c_double
+        lib.refh_calc_dt.argtypes = [C.c_double]
+        lib.refh_stage.argtypes = [C.c_int, C.c_double]
+        lib.refh_step.restype = C.c_double
+        lib.refh_step.argtypes = [C.c_double, C.c_double]
+        lib.prj_riemann_hllc.argtypes = [_dp, _dp] + [C.c_double] * 4 + [C.c_void_p, _dp, _dp] + \
+            [C.c_double] * 3
+        if mhd:
+            lib.refh_set_bf.argtypes = [C.c_int, _dp, _dp, _dp]
+            lib.refh_get_bf.argtypes = [C.c_int, _dp, _dp, _dp]
+            lib.refh_get_emf.argtypes = [_dp, _dp, _dp]
+            lib.refh_emf_upwind.restype = C.c_double
+            lib.refh_emf_upwind.argtypes = [C.c_int, _dp, _dp, _dp]
+            lib.prj_riemann_hlld.argtypes = [_dp, _dp] + [C.c_double] * 4 + \
+                [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp] + [C.c_double] * 3
+        _ref_libs[key] = lib
+    return _ref_libs[key]
+
+
+class RefSim:
+    """The serial reference build driven like src/main.c (one mesh per process)."""
+
+    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0):
+        self.lib = ref_lib(mhd)
+        nb = [int(v) // 16 for v in n]
+        xl = np.ascontiguousarray(xlim, dtype=np.float64)
+        bcs = (C.c_int * 6)(*bc)
+        rc = self.lib.refh_init(nb[0], nb[1], nb[2], _ptr(xl), bcs, float(gamma), float(eta))
+        if rc != 0:
+            raise RuntimeError(f"refh_init failed ({rc})")
+        self.n = tuple(int(v) for v in n)
+        self.mhd = bool(mhd)
+        self.nh = 9 if mhd else 6
+        dx = np.zeros(3)
+        area = np.zeros(3)
+        vol = np.zeros(1)
+        self.lib.refh_geometry(_ptr(dx), _ptr(area), _ptr(vol))
+        self.dx, self.area, self.vol = dx, area, float(vol[0])
+
+    def set_state(self, W, bf=None):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        self.lib.refh_set_prim(0, _ptr(W))
+        self.lib.refh_set_prim(1, _ptr(W))
+        if self.mhd:
+            bf = [np.ascontiguousarray(b, dtype=np.float64) for b in bf]
+            self.lib.refh_set_bf(0, *[_ptr(b) for b in bf])
+            self.lib.refh_set_bf(1, *[_ptr(b) for b in bf])
+        self.lib.refh_prepare()
+
+    def get_prim(self, slot):
+        W = np.empty((self.nh,) + self.n)
+        self.lib.refh_get_prim(int(slot), _ptr(W))
+        return W
+
+    def get_bf(self, slot):
+        bf = [np.empty(s) for s in face_shapes(self.n)]
+        self.lib.refh_get_bf(int(slot), *[_ptr(b) for b in bf])
+        return bf
+
+    def get_emf(self):
+        e = [np.empty(s) for s in edge_shapes(self.n)]
+        self.lib.refh_get_emf(*[_ptr(a) for a in e])
+        return e
+
+    def calc_dt(self, cfl):
+        return self.lib.refh_calc_dt(float(cfl))
+
+    def stage(self, s, dt):
+        self.lib.refh_stage(int(s), float(dt))
+
+    def step(self, cfl, dt_factor=1.2):
+        return self.lib.refh_step(float(cfl), float(dt_factor))
+
+    def close(self):
+        self.lib.refh_destroy()
