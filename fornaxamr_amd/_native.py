@@ -1,0 +1,84 @@
+"""ctypes binding of the C ABI in include/fornax_gpu.h.
+
+The product path has no fallback: if ``libfornax_gpu.so`` is missing or no
+GPU is visible, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import build as _build
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("xlim", C.c_double * 6), ("bc", C.c_int * 6),
+                ("gamma", C.c_double), ("mhd", C.c_int), ("eta", C.c_double)]
+
+
+class MeshView(C.Structure):
+    _fields_ = [("W", C.c_void_p * 2), ("bf", C.c_void_p * 3), ("vstride", C.c_longlong),
+                ("sx", C.c_longlong), ("sy", C.c_longlong), ("origin", C.c_longlong),
+                ("n", C.c_int * 3), ("nh", C.c_int)]
+
+
+class FxError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return _build.LIB
+
+
+def load():
+    """Load the gfx950 library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise FxError(f"{path} is missing: run fornaxamr_amd.build.build() (hipcc, gfx950) first")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.fx_last_error.restype = C.c_char_p
+    lib.fx_version.restype = C.c_char_p
+    lib.fx_mesh_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
+    lib.fx_mesh_destroy.argtypes = [vp]
+    lib.fx_mesh_set_stream.argtypes = [vp, vp]
+    lib.fx_mesh_set_prim.argtypes = [vp, _dp]
+    lib.fx_mesh_get_prim.argtypes = [vp, C.c_int, _dp]
+    lib.fx_mesh_set_bf.argtypes = [vp, _dp, _dp, _dp]
+    lib.fx_mesh_get_bf.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    lib.fx_calc_dt.argtypes = [vp, C.c_double, _dp]
+    lib.fx_stage.argtypes = [vp, C.c_int, C.c_double]
+    lib.fx_step_async.argtypes = [vp, C.c_double, C.c_double]
+    lib.fx_step.argtypes = [vp, C.c_double, C.c_double, _dp]
+    lib.fx_sync.argtypes = [vp]
+    lib.fx_last_dt.argtypes = [vp, _dp]
+    lib.fx_mesh_view_get.argtypes = [vp, C.POINTER(MeshView)]
+    lib.fx_mesh_set_timing.argtypes = [vp, C.c_int]
+    lib.fx_mesh_kernel_time.argtypes = [vp, C.c_int, _dp, C.POINTER(C.c_int)]
+    lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
+    lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
+    lib.fx_emf_upwind_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise FxError(f"fornax_gpu error {rc}: {load().fx_last_error().decode()}")
+
+
+# the exported symbols every build must provide (include/fornax_gpu.h)
+EXPORTS = [
+    "fx_mesh_create", "fx_mesh_destroy", "fx_mesh_set_stream", "fx_mesh_set_prim",
+    "fx_mesh_get_prim", "fx_mesh_set_bf", "fx_mesh_get_bf", "fx_calc_dt", "fx_stage",
+    "fx_step_async", "fx_step", "fx_sync", "fx_last_dt", "fx_mesh_view_get",
+    "fx_mesh_set_timing", "fx_mesh_kernel_time",
+    "fx_riemann_hllc_batch", "fx_riemann_hlld_batch", "fx_emf_upwind_batch",
+    "fx_last_error", "fx_version",
+]

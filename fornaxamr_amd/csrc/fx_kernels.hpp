@@ -1,0 +1,63 @@
+// Kernel argument blocks and launchers shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fx_layout.hpp"
+
+namespace fx {
+
+struct HydroStageArgs {
+    Geom g;
+    const double* Win;      // stage input W (padded layout, 6 variables)
+    const double* W0;       // W^n for the stage-2 blend (== Win for stage 1)
+    double* Wout;           // stage output (may alias W0 for stage 2)
+    const double* dt_ptr;   // device dt, or null to use dt
+    double dt;
+};
+
+struct MhdStageArgs {
+    Geom g;
+    const double* Win;      // 9 variables
+    const double* W0;
+    double* Wout;
+    const double* bf_in;    // face fields of the stage input (3 components)
+    const double* bf0;      // face fields at t^n (stage-2 blend)
+    double* bf_out;
+    double* U1;             // scratch: hydro part of the updated conserved state (6 vars)
+    double* fdat;           // scratch: face EMFs and normal velocities (3 dirs x 3)
+    int* err;               // device error word: [code, dir, x, y, z]
+    const double* dt_ptr;
+    double dt;
+};
+
+// fill the two-cell ghost shell of the domain (periodic / outflow / reflect;
+// MHD: cell B from the clamped outflow face fields)
+hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s);
+
+hipError_t launch_hydro_stage(const HydroStageArgs& a, int stage, hipStream_t s);
+
+hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s);
+
+// cell-centred B of the active cells from the faces (prj_mhd_bf2bc)
+hipError_t launch_bf2bc(const Geom& g, double* W, const double* bf, hipStream_t s);
+
+// CFL time step: dt_out[0] = min over active cells of cfl / denom; if
+// dt_prev is non-null and *dt_prev > 0, also applies min(dt, factor * dt_prev)
+// and stores the result to *dt_prev.
+hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* scratch,
+                          double* dt_out, double* dt_prev, double factor, hipStream_t s);
+
+// per-face / per-edge batched entry points (global-memory SoA arrays)
+hipError_t launch_hllc_batch(int n, const double* WL, const double* WR, const double* pL,
+                             const double* pR, const double* gL, const double* gR,
+                             const double* du, const double* dv, const double* dw,
+                             double* flux, double* vface, hipStream_t s);
+hipError_t launch_hlld_batch(int n, const double* WL, const double* WR, const double* pL,
+                             const double* pR, const double* gL, const double* gR,
+                             const double* bn, double* flux, double* vface, double* bv1,
+                             double* bv2, int* status, hipStream_t s);
+hipError_t launch_emf_batch(int n, const double* ef, const double* ec, const double* vn,
+                            double* emf, hipStream_t s);
+
+}  // namespace fx

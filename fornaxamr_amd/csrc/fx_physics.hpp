@@ -1,0 +1,396 @@
+// Per-cell / per-face / per-edge physics of the RK2 Godunov stage, as device
+// functions for gfx950.  FP64 throughout.
+//
+// Every expression keeps the operation order and association of the reference
+// (cited per function) and the library is built with -ffp-contract=off, so the
+// results are bit-identical to the reference CPU path whenever the FP64
+// division and square root round correctly (they do on gfx950).  Min/max are
+// written as the reference's ternaries, not fmin/fmax, to keep signed-zero and
+// NaN behaviour identical.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fx {
+
+// primitive / conserved variable slots (src/prj_defs.h:466-494)
+enum { RHO = 0, V1 = 1, V2 = 2, V3 = 3, EINT = 4, YE = 5, B1 = 6, B2 = 7, B3 = 8 };
+enum { MOM1 = 1, MOM2 = 2, MOM3 = 3, ETOT = 4 };
+
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// MC limited slope (src/prj_reconstruct.h:355-375).
+__device__ __forceinline__ double mc_slope(double qm, double q0, double qp)
+{
+    double sl = q0 - qm;
+    double sr = qp - q0;
+    double c0 = 2.0 * sl;
+    double c1 = sl + 0.5 * (sr - sl);
+    double c2 = 2.0 * sr;
+    double mn = c0 < c1 ? c0 : c1;
+    double mx = c0 > c1 ? c0 : c1;
+    mn = mn < c2 ? mn : c2;
+    mx = mx > c2 ? mx : c2;
+    double slope = sl > 0.0 ? mn : mx;
+    return (sl * sr > 0.0) ? slope : 0.0;
+}
+
+// Half slope as the flux assembly uses it (src/prj_flux.c:410-418).
+__device__ __forceinline__ double mc_half(double qm, double q0, double qp)
+{
+    return 0.5 * mc_slope(qm, q0, qp);
+}
+
+// Shock-detector pressure-jump limiter (src/prj_riemann.c:8-17).
+__device__ __forceinline__ double theta_limiter(double cfmax, double du, double dv, double dw)
+{
+    double t = (-dmin(du, 0.0) + cfmax) / (-dmin(dmin(dv, dw), 0.0) + cfmax);
+    t *= t;
+    return dmin(1.0, t * t);
+}
+
+// HLLC with the theta-limited contact speed (src/prj_riemann.c:1433-1599),
+// hydro build (6 variables).  W* are face states in the rotated frame.
+__device__ __forceinline__ void hllc(const double* WL, const double* WR, double pL, double pR,
+                                     double gL, double gR, double du, double dv, double dw,
+                                     double* flux, double* vface)
+{
+    // prj_riemann_state for both sides
+    double rL = WL[RHO], uL = WL[V1], vL = WL[V2], wL = WL[V3], yL = WL[YE];
+    double rR = WR[RHO], uR = WR[V1], vR = WR[V2], wR = WR[V3], yR = WR[YE];
+    double eL = rL * WL[EINT] + 0.5 * rL * (uL * uL + vL * vL + wL * wL);
+    double eR = rR * WR[EINT] + 0.5 * rR * (uR * uR + vR * vR + wR * wR);
+    double csL = sqrt(gL * pL / rL);
+    double csR = sqrt(gR * pR / rR);
+    double UL1 = rL * uL, UR1 = rR * uR;
+
+    double SL = dmin(uL - csL, uR - csR);
+    double SR = dmax(uL + csL, uR + csR);
+    double cfmax = dmax(csL, csR);
+    double theta = theta_limiter(cfmax, du, dv, dw);
+    double SM = (theta * (pR - pL) + UL1 * (SL - uL) - UR1 * (SR - uR)) /
+                (rL * (SL - uL) - rR * (SR - uR));
+
+    if (0.0 <= SL) {
+        flux[0] = rL * uL;
+        flux[1] = rL * uL * uL + pL;
+        flux[2] = rL * uL * vL;
+        flux[3] = rL * uL * wL;
+        flux[4] = (eL + pL) * uL;
+        flux[5] = rL * yL * uL;
+        vface[0] = uL; vface[1] = vL; vface[2] = wL;
+        return;
+    }
+    if (SR <= 0.0) {
+        flux[0] = rR * uR;
+        flux[1] = rR * uR * uR + pR;
+        flux[2] = rR * uR * vR;
+        flux[3] = rR * uR * wR;
+        flux[4] = (eR + pR) * uR;
+        flux[5] = rR * yR * uR;
+        vface[0] = uR; vface[1] = vR; vface[2] = wR;
+        return;
+    }
+    vface[0] = SM;
+    // Select the upwind side once; the star-state formulas are identical in
+    // form for both sides (Toro 2009, eqs. 10.33-10.36).
+    const bool left = 0.0 <= SM;
+    double r = left ? rL : rR;
+    double u = left ? uL : uR;
+    double v = left ? vL : vR;
+    double w = left ? wL : wR;
+    double y = left ? yL : yR;
+    double e = left ? eL : eR;
+    double p = left ? pL : pR;
+    double S = left ? SL : SR;
+    vface[1] = v;
+    vface[2] = w;
+    double rho_star = r * (S - u) / (S - SM);
+    double e_over_rho = e / r + (SM - u) * (SM + p / (r * (S - u)));
+    double Fr = r * u;
+    double Fm1 = r * u * u + p;
+    double Fm2 = r * u * v;
+    double Fm3 = r * u * w;
+    double Fe = (e + p) * u;
+    double Fy = r * y * u;
+    flux[0] = Fr + S * (rho_star - r);
+    flux[1] = Fm1 + S * (rho_star * SM - r * u);
+    flux[2] = Fm2 + S * (rho_star * v - r * v);
+    flux[3] = Fm3 + S * (rho_star * w - r * w);
+    flux[4] = Fe + S * (rho_star * e_over_rho - e);
+    flux[5] = Fy + S * (rho_star * y - r * y);
+}
+
+// ---- HLLD (src/prj_riemann.c:19-440) --------------------------------------
+
+// Error codes mirror the reference abort sites.
+enum HlldError {
+    HLLD_OK = 0,
+    HLLD_BAD_RHO = 11,       // src/prj_riemann.c:159-161
+    HLLD_BAD_P = 12,         // :162-164
+    HLLD_BAD_GAMMA = 13,     // :165-167
+    HLLD_BAD_DISC = 14,      // :126-128
+    HLLD_BAD_CF = 15,        // :131-133
+    HLLD_BAD_ORDER = 16,     // :361-363
+    HLLD_BAD_CONTACT = 17,   // :381-384
+    HLLD_COINCIDE = 18,      // :198-200
+    HLLD_BAD_STAR_RHO = 19,  // :203-205
+    HLLD_BAD_DSS = 20        // :247-249
+};
+
+struct HlldSide {
+    double rho, vx, vy, vz, by, bz, p, g, ye, e, pt;
+};
+
+struct HlldStar {
+    double rho, vx, vy, vz, by, bz, ye, e;
+};
+
+// conserved vector of a state (src/prj_riemann.c:81-93)
+__device__ __forceinline__ void hlld_cons(double rho, double vx, double vy, double vz, double bx,
+                                          double by, double bz, double ye, double e, double* U)
+{
+    U[0] = rho;
+    U[1] = rho * vx;
+    U[2] = rho * vy;
+    U[3] = rho * vz;
+    U[4] = e;
+    U[5] = rho * ye;
+    U[6] = bx;
+    U[7] = by;
+    U[8] = bz;
+}
+
+// physical flux (src/prj_riemann.c:95-115)
+__device__ __forceinline__ void hlld_flux(const HlldSide& s, double bx, double* F)
+{
+    double b2 = bx * bx + s.by * s.by + s.bz * s.bz;
+    double pt = s.p + 0.5 * b2;
+    double vdotb = s.vx * bx + s.vy * s.by + s.vz * s.bz;
+    F[0] = s.rho * s.vx;
+    F[1] = s.rho * s.vx * s.vx + pt - bx * bx;
+    F[2] = s.rho * s.vx * s.vy - bx * s.by;
+    F[3] = s.rho * s.vx * s.vz - bx * s.bz;
+    F[4] = (s.e + pt) * s.vx - bx * vdotb;
+    F[5] = s.rho * s.ye * s.vx;
+    F[6] = 0.0;
+    F[7] = s.by * s.vx - bx * s.vy;
+    F[8] = s.bz * s.vx - bx * s.vz;
+}
+
+// state from primitives (src/prj_riemann.c:142-179)
+__device__ __forceinline__ int hlld_side(const double* W, double p, double g, double bn, HlldSide& s)
+{
+    s.rho = W[RHO];
+    s.vx = W[V1];
+    s.vy = W[V2];
+    s.vz = W[V3];
+    s.by = W[B2];
+    s.bz = W[B3];
+    s.p = p;
+    s.g = g;
+    s.ye = W[YE];
+    if (s.rho <= 0.0) return HLLD_BAD_RHO;
+    if (s.p <= 0.0) return HLLD_BAD_P;
+    if (s.g <= 0.0) return HLLD_BAD_GAMMA;
+    double kinetic = 0.5 * s.rho * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    double magnetic = 0.5 * (bn * bn + s.by * s.by + s.bz * s.bz);
+    s.e = s.rho * W[EINT] + kinetic + magnetic;
+    s.pt = s.p + magnetic;
+    return HLLD_OK;
+}
+
+// fast magnetosonic speed (src/prj_riemann.c:117-140)
+__device__ __forceinline__ int hlld_fast(const HlldSide& s, double bx, double& cf)
+{
+    double a2 = s.g * s.p / s.rho;
+    double bx2_rho = bx * bx / s.rho;
+    double b2_rho = (bx * bx + s.by * s.by + s.bz * s.bz) / s.rho;
+    double disc = (a2 + b2_rho) * (a2 + b2_rho) - 4.0 * a2 * bx2_rho;
+    if (disc < 0.0) {
+        if (disc < -1.0e-12 * (a2 + b2_rho) * (a2 + b2_rho)) return HLLD_BAD_DISC;
+        disc = 0.0;
+    }
+    double cf2 = 0.5 * (a2 + b2_rho + sqrt(disc));
+    if (cf2 < 0.0 || !isfinite(cf2)) return HLLD_BAD_CF;
+    cf = sqrt(cf2);
+    return HLLD_OK;
+}
+
+// outer (single-star) state (src/prj_riemann.c:187-232)
+__device__ __forceinline__ int hlld_outer(const HlldSide& s, double bx, double S, double SM,
+                                          double pt_star, HlldStar& st)
+{
+    double alpha = S - s.vx;
+    double smdiff = S - SM;
+    if (fabs(smdiff) <= 1.0e-14 * (fabs(S) + fabs(SM) + 1.0)) return HLLD_COINCIDE;
+    st.rho = s.rho * alpha / smdiff;
+    if (st.rho <= 0.0 || !isfinite(st.rho)) return HLLD_BAD_STAR_RHO;
+    st.vx = SM;
+    st.ye = s.ye;
+    double denom = s.rho * alpha * smdiff - bx * bx;
+    if (fabs(denom) < 1.0e-8 * pt_star) {
+        st.vy = s.vy;
+        st.vz = s.vz;
+        st.by = s.by;
+        st.bz = s.bz;
+    } else {
+        double q = (SM - s.vx) / denom;
+        double mag = (s.rho * alpha * alpha - bx * bx) / denom;
+        st.vy = s.vy - bx * s.by * q;
+        st.vz = s.vz - bx * s.bz * q;
+        st.by = s.by * mag;
+        st.bz = s.bz * mag;
+    }
+    double vdotb = s.vx * bx + s.vy * s.by + s.vz * s.bz;
+    double vsdotbs = st.vx * bx + st.vy * st.by + st.vz * st.bz;
+    st.e = (alpha * s.e - s.pt * s.vx + pt_star * SM + bx * (vdotb - vsdotbs)) / smdiff;
+    return HLLD_OK;
+}
+
+// HLLD flux (src/prj_riemann.c:321-440).  WL/WR are rotated face states with
+// the normal field slot already set to bn.  Outputs the 9-variable flux, the
+// face velocity and the face EMFs Bv1 = bz*vx - bx*vz, Bv2 = bx*vy - by*vx of
+// the selected fan state (src/prj_riemann.c:305-319).
+__device__ __forceinline__ int hlld(const double* WL, const double* WR, double pL, double pR,
+                                 double gL, double gR, double bn, double* flux, double* vface,
+                                 double* bv1, double* bv2)
+{
+    HlldSide L, R;
+    int rc;
+    if ((rc = hlld_side(WL, pL, gL, bn, L)) != 0) return rc;
+    if ((rc = hlld_side(WR, pR, gR, bn, R)) != 0) return rc;
+    double cfL, cfR;
+    if ((rc = hlld_fast(L, bn, cfL)) != 0) return rc;
+    if ((rc = hlld_fast(R, bn, cfR)) != 0) return rc;
+    double SL = dmin(L.vx - cfL, R.vx - cfR);
+    double SR = dmax(L.vx + cfL, R.vx + cfR);
+    if (!(SL < SR)) return HLLD_BAD_ORDER;
+
+    double UL[9], UR[9], FL[9], FR[9];
+    if (0.0 <= SL || SR <= 0.0) {
+        const HlldSide& s = 0.0 <= SL ? L : R;
+        hlld_flux(s, bn, flux);
+        vface[0] = s.vx; vface[1] = s.vy; vface[2] = s.vz;
+        *bv1 = s.bz * s.vx - bn * s.vz;
+        *bv2 = bn * s.vy - s.by * s.vx;
+        return HLLD_OK;
+    }
+    double denom = R.rho * (SR - R.vx) - L.rho * (SL - L.vx);
+    if (fabs(denom) <= 1.0e-14 * (fabs(R.rho * (SR - R.vx)) + fabs(L.rho * (SL - L.vx)) + 1.0))
+        return HLLD_BAD_CONTACT;
+    double SM = (R.rho * R.vx * (SR - R.vx) - L.rho * L.vx * (SL - L.vx) + (L.pt - R.pt)) / denom;
+    double ptl = L.pt + L.rho * (SL - L.vx) * (SM - L.vx);
+    double ptr = R.pt + R.rho * (SR - R.vx) * (SM - R.vx);
+    double pt_star = 0.5 * (ptl + ptr);
+    hlld_flux(L, bn, FL);
+    hlld_flux(R, bn, FR);
+    hlld_cons(L.rho, L.vx, L.vy, L.vz, bn, L.by, L.bz, L.ye, L.e, UL);
+    hlld_cons(R.rho, R.vx, R.vy, R.vz, bn, R.by, R.bz, R.ye, R.e, UR);
+    if (pt_star <= 0.0 || !isfinite(pt_star)) {
+        double inv = 1.0 / (SR - SL);
+        double Uh[9];
+        for (int v = 0; v < 9; ++v) {
+            flux[v] = (SR * FL[v] - SL * FR[v] + SL * SR * (UR[v] - UL[v])) * inv;
+            Uh[v] = (SR * UR[v] - SL * UL[v] + FL[v] - FR[v]) * inv;
+        }
+        double rho = Uh[0];
+        double vx = Uh[1] / rho, vy = Uh[2] / rho, vz = Uh[3] / rho;
+        vface[0] = vx; vface[1] = vy; vface[2] = vz;
+        *bv1 = Uh[8] * vx - Uh[6] * vz;
+        *bv2 = Uh[6] * vy - Uh[7] * vx;
+        return HLLD_OK;
+    }
+    const bool bn_small = (0.5 * bn * bn < 1.0e-8 * pt_star);
+    // One side is upwind of the contact; the other side's star state is only
+    // needed for the double-star (rotational) states.
+    const bool left = 0.0 <= SM;
+    const HlldSide& A = left ? L : R;        // upwind outer side
+    const double SA = left ? SL : SR;
+    const double* FA = left ? FL : FR;
+    const double* UA = left ? UL : UR;
+    HlldStar As;
+    if ((rc = hlld_outer(A, bn, SA, SM, pt_star, As)) != 0) return rc;
+    double sqA = sqrt(As.rho);
+    double SAs = left ? SM - fabs(bn) / sqA : SM + fabs(bn) / sqA;
+    double UAs[9];
+    hlld_cons(As.rho, As.vx, As.vy, As.vz, bn, As.by, As.bz, As.ye, As.e, UAs);
+    if ((left ? 0.0 <= SAs : SAs <= 0.0) || bn_small) {
+        for (int v = 0; v < 9; ++v) flux[v] = FA[v] + SA * (UAs[v] - UA[v]);
+        vface[0] = As.vx; vface[1] = As.vy; vface[2] = As.vz;
+        *bv1 = As.bz * As.vx - bn * As.vz;
+        *bv2 = bn * As.vy - As.by * As.vx;
+        return HLLD_OK;
+    }
+    // double-star state (src/prj_riemann.c:234-282)
+    const HlldSide& Bo = left ? R : L;
+    const double SB = left ? SR : SL;
+    HlldStar Bs;
+    if ((rc = hlld_outer(Bo, bn, SB, SM, pt_star, Bs)) != 0) return rc;
+    const HlldStar& Ls = left ? As : Bs;
+    const HlldStar& Rs = left ? Bs : As;
+    double sl = sqrt(Ls.rho);
+    double sr = sqrt(Rs.rho);
+    if (sl + sr <= 0.0) return HLLD_BAD_DSS;
+    double inv_sum = 1.0 / (sl + sr);
+    double sgn = bn < 0.0 ? -1.0 : 1.0;
+    double vy = (sl * Ls.vy + sr * Rs.vy + (Rs.by - Ls.by) * sgn) * inv_sum;
+    double vz = (sl * Ls.vz + sr * Rs.vz + (Rs.bz - Ls.bz) * sgn) * inv_sum;
+    double by = (sl * Rs.by + sr * Ls.by + sl * sr * (Rs.vy - Ls.vy) * sgn) * inv_sum;
+    double bz = (sl * Rs.bz + sr * Ls.bz + sl * sr * (Rs.vz - Ls.vz) * sgn) * inv_sum;
+    double vbss = (left ? Ls.vx : Rs.vx) * bn + vy * by + vz * bz;
+    double ess;
+    if (left) {
+        double vbl = Ls.vx * bn + Ls.vy * Ls.by + Ls.vz * Ls.bz;
+        ess = Ls.e - sl * sgn * (vbl - vbss);
+    } else {
+        double vbr = Rs.vx * bn + Rs.vy * Rs.by + Rs.vz * Rs.bz;
+        ess = Rs.e + sr * sgn * (vbr - vbss);
+    }
+    double Uss[9];
+    hlld_cons(As.rho, As.vx, vy, vz, bn, by, bz, As.ye, ess, Uss);
+    for (int v = 0; v < 9; ++v) flux[v] = FA[v] + SA * (UAs[v] - UA[v]) + SAs * (Uss[v] - UAs[v]);
+    vface[0] = As.vx; vface[1] = vy; vface[2] = vz;
+    *bv1 = bz * As.vx - bn * vz;
+    *bv2 = bn * vy - by * As.vx;
+    return HLLD_OK;
+}
+
+// Upwind CT edge EMF without the resistive term (src/prj_mhd.c:651-683).
+__device__ __forceinline__ double emf_upwind(const double* ef, const double* ec, const double* vn)
+{
+    double emf = 0.25 * (ef[0] + ef[1] + ef[2] + ef[3]);
+    if (vn[0] > 0.0)      emf += 0.25 * (ef[3] - ec[3]);
+    else if (vn[0] < 0.0) emf += 0.25 * (ef[1] - ec[0]);
+    else                  emf += 0.125 * (ef[3] - ec[3] + ef[1] - ec[0]);
+    if (vn[1] > 0.0)      emf += 0.25 * (ef[2] - ec[1]);
+    else if (vn[1] < 0.0) emf += 0.25 * (ef[0] - ec[0]);
+    else                  emf += 0.125 * (ef[2] - ec[1] + ef[0] - ec[0]);
+    if (vn[2] > 0.0)      emf += 0.25 * (ef[3] - ec[2]);
+    else if (vn[2] < 0.0) emf += 0.25 * (ef[1] - ec[1]);
+    else                  emf += 0.125 * (ef[3] - ec[2] + ef[1] - ec[1]);
+    if (vn[3] > 0.0)      emf += 0.25 * (ef[2] - ec[2]);
+    else if (vn[3] < 0.0) emf += 0.25 * (ef[0] - ec[3]);
+    else                  emf += 0.125 * (ef[2] - ec[2] + ef[0] - ec[3]);
+    return emf;
+}
+
+// Anomalous resistivity mask of one cell (src/prj_timeint.c:230-263).  The
+// log factors are positive only above the thresholds, so they are evaluated
+// only there; the value is identical to max(0, log(.)).
+__device__ __forceinline__ double eta_mask(double rho, double ye)
+{
+    double rf = 0.0, yf = 0.0;
+    if (isfinite(rho) && rho > 0.0) {
+        double a = rho / 1.0e13;
+        if (a > 1.0) rf = dmax(0.0, log(a));
+    }
+    if (isfinite(ye) && ye > 0.0) {
+        double b = ye / 0.25;
+        if (b > 1.0) yf = dmax(0.0, log(b));
+    }
+    return 1.0e12 * rf * yf;
+}
+
+}  // namespace fx

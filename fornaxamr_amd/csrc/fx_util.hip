@@ -1,0 +1,273 @@
+// Domain ghost fill, CFL time step and the per-face / per-edge batched entry
+// points.
+#include <hip/hip_runtime.h>
+
+#include "fx_kernels.hpp"
+#include "fx_physics.hpp"
+
+namespace fx {
+
+namespace {
+
+// Ghost mapping of one axis: periodic wrap (the wrapped neighbour of
+// src/prj_mesh.c:1597-1645) or the mirror copy of the physical boundary
+// (src/prj_boundary.c:413-487: src = -a-1 / 2N-1-a), reflect negating the
+// normal velocity.
+__device__ __forceinline__ int map_axis(const Geom& g, int axis, int n, int c, int& flip)
+{
+    if (c >= 0 && c < n) return c;
+    int t = c < 0 ? g.bc[2 * axis] : g.bc[2 * axis + 1];
+    if (t == 3) return c < 0 ? c + n : c - n;
+    if (t == 1) flip ^= 1 << axis;
+    return c < 0 ? -c - 1 : 2 * n - 1 - c;
+}
+
+// outflow face-field ghost value: nearest storage face inside the domain
+// (src/prj_boundary.c:902-995)
+__device__ __forceinline__ double bf_clamped(const Geom& g, const double* bf, int dir, int x,
+                                             int y, int z)
+{
+    int hx = g.nx - 1 + (dir == 0), hy = g.ny - 1 + (dir == 1), hz = g.nz - 1 + (dir == 2);
+    x = x < 0 ? 0 : (x > hx ? hx : x);
+    y = y < 0 ? 0 : (y > hy ? hy : y);
+    z = z < 0 ? 0 : (z > hz ? hz : z);
+    return bf[dir * g.fvstride + cell_off(g, x, y, z)];
+}
+
+__global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const double* __restrict__ bf,
+                                     int64_t nx_part, int64_t ny_part, int64_t total)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    const int nxp = g.nx + 2 * NG, nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
+    int x, y, z;
+    if (id < nx_part) {
+        int64_t r = id;
+        z = (int)(r % nzp) - NG; r /= nzp;
+        y = (int)(r % nyp) - NG; r /= nyp;
+        int s = (int)r;                     // 0..3
+        x = s < 2 ? s - NG : g.nx + (s - 2);
+    } else if (id < nx_part + ny_part) {
+        int64_t r = id - nx_part;
+        z = (int)(r % nzp) - NG; r /= nzp;
+        int s = (int)(r % 4); r /= 4;
+        y = s < 2 ? s - NG : g.ny + (s - 2);
+        x = (int)r;
+    } else {
+        int64_t r = id - nx_part - ny_part;
+        int s = (int)(r % 4); r /= 4;
+        z = s < 2 ? s - NG : g.nz + (s - 2);
+        y = (int)(r % g.ny); r /= g.ny;
+        x = (int)r;
+    }
+    int flip = 0;
+    int mx = map_axis(g, 0, g.nx, x, flip);
+    int my = map_axis(g, 1, g.ny, y, flip);
+    int mz = map_axis(g, 2, g.nz, z, flip);
+    const int64_t src = cell_off(g, mx, my, mz);
+    const int64_t dst = cell_off(g, x, y, z);
+    const int64_t vs = g.vstride;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+        double val = W[v * vs + src];
+        if (v >= 1 && v <= 3 && (flip & (1 << (v - 1)))) val = -val;
+        W[v * vs + dst] = val;
+    }
+    if (g.mhd) {
+        // cell B of a ghost cell is the average of its (ghost) faces, as
+        // prj_mhd_bf2bc_all_mesh does after the ghost exchange
+        // (src/prj_boundary.c:1234-1239; src/prj_mhd.c:555-603)
+        double b1 = 0.5 * (bf_clamped(g, bf, 0, x, y, z) + bf_clamped(g, bf, 0, x + 1, y, z));
+        double b2 = 0.5 * (bf_clamped(g, bf, 1, x, y, z) + bf_clamped(g, bf, 1, x, y + 1, z));
+        double b3 = 0.5 * (bf_clamped(g, bf, 2, x, y, z) + bf_clamped(g, bf, 2, x, y, z + 1));
+        W[6 * vs + dst] = b1 / 1.0;
+        W[7 * vs + dst] = b2 / 1.0;
+        W[8 * vs + dst] = b3 / 1.0;
+    }
+}
+
+__global__ void dt_init_kernel(unsigned long long* s)
+{
+    s[0] = (unsigned long long)__double_as_longlong(1.0e99);
+}
+
+// src/prj_timeint.c:2144-2256 (sound speed only, also for MHD)
+__global__ void __launch_bounds__(256) dt_reduce_kernel(Geom g, const double* __restrict__ W,
+                                                        double cfl, unsigned long long* s)
+{
+    __shared__ double red[256];
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    double best = 1.0e99;
+    const double gm1 = g.gamma - 1.0;
+    for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+         id += (int64_t)gridDim.x * blockDim.x) {
+        int z = (int)(id % g.nz);
+        int64_t r = id / g.nz;
+        int y = (int)(r % g.ny);
+        int x = (int)(r / g.ny);
+        int64_t o = cell_off(g, x, y, z);
+        double rho = W[o];
+        double p = gm1 * rho * W[4 * g.vstride + o];
+        double cs = sqrt(g.gamma * p / rho);
+        double denom = (fabs(W[g.vstride + o]) + cs) / g.dx[0] +
+                       (fabs(W[2 * g.vstride + o]) + cs) / g.dx[1] +
+                       (fabs(W[3 * g.vstride + o]) + cs) / g.dx[2];
+        double dt_cell = cfl / denom;
+        if (dt_cell < best) best = dt_cell;
+    }
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            double o = red[threadIdx.x + w];
+            if (o < red[threadIdx.x]) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        // dt > 0: IEEE order of positive doubles is the order of their bits
+        atomicMin(s, (unsigned long long)__double_as_longlong(red[0]));
+    }
+}
+
+// dt = min(dt_new, factor * dt_prev) after the first step (src/main.c:517-522)
+__global__ void dt_finalize_kernel(const unsigned long long* s, double* dt_out, double* dt_prev,
+                                   double factor)
+{
+    double dt_new = __longlong_as_double((long long)s[0]);
+    double dt = dt_new;
+    if (dt_prev != nullptr) {
+        double prev = dt_prev[0];
+        if (prev > 0.0) {
+            double lim = factor * prev;
+            dt = dt_new <= lim ? dt_new : lim;
+        }
+        dt_prev[0] = dt;
+    }
+    dt_out[0] = dt;
+}
+
+__global__ void hllc_batch_kernel(int n, const double* WL, const double* WR, const double* pL,
+                                  const double* pR, const double* gL, const double* gR,
+                                  const double* du, const double* dv, const double* dw,
+                                  double* flux, double* vface)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    double wl[6], wr[6], F[6], vf[3];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+        wl[v] = WL[(int64_t)v * n + f];
+        wr[v] = WR[(int64_t)v * n + f];
+    }
+    hllc(wl, wr, pL[f], pR[f], gL[f], gR[f], du[f], dv[f], dw[f], F, vf);
+#pragma unroll
+    for (int v = 0; v < 6; ++v) flux[(int64_t)v * n + f] = F[v];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) vface[(int64_t)v * n + f] = vf[v];
+}
+
+__global__ void hlld_batch_kernel(int n, const double* WL, const double* WR, const double* pL,
+                                  const double* pR, const double* gL, const double* gR,
+                                  const double* bn, double* flux, double* vface, double* bv1,
+                                  double* bv2, int* status)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    double wl[9], wr[9], F[9], vf[3], e1 = 0.0, e2 = 0.0;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        wl[v] = WL[(int64_t)v * n + f];
+        wr[v] = WR[(int64_t)v * n + f];
+    }
+    wl[B1] = bn[f];
+    wr[B1] = bn[f];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) F[v] = 0.0;
+    vf[0] = vf[1] = vf[2] = 0.0;
+    int rc = hlld(wl, wr, pL[f], pR[f], gL[f], gR[f], bn[f], F, vf, &e1, &e2);
+    status[f] = rc;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) flux[(int64_t)v * n + f] = F[v];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) vface[(int64_t)v * n + f] = vf[v];
+    bv1[f] = e1;
+    bv2[f] = e2;
+}
+
+__global__ void emf_batch_kernel(int n, const double* ef, const double* ec, const double* vn,
+                                 double* emf)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    double a[4], c[4], v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        a[q] = ef[(int64_t)q * n + f];
+        c[q] = ec[(int64_t)q * n + f];
+        v[q] = vn[(int64_t)q * n + f];
+    }
+    emf[f] = emf_upwind(a, c, v);
+}
+
+inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s)
+{
+    const int64_t nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
+    const int64_t nx_part = 4 * nyp * nzp;
+    const int64_t ny_part = 4 * (int64_t)g.nx * nzp;
+    const int64_t nz_part = 4 * (int64_t)g.nx * g.ny;
+    const int64_t total = nx_part + ny_part + nz_part;
+    hipLaunchKernelGGL(boundary_fill_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, g, W,
+                       bf, nx_part, ny_part, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* scratch,
+                          double* dt_out, double* dt_prev, double factor, hipStream_t s)
+{
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(scratch);
+    hipLaunchKernelGGL(dt_init_kernel, dim3(1), dim3(1), 0, s, acc);
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    unsigned blocks = grid_for(n, 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(dt_reduce_kernel, dim3(blocks), dim3(256), 0, s, g, W, cfl, acc);
+    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, s, acc, dt_out, dt_prev, factor);
+    return hipGetLastError();
+}
+
+hipError_t launch_hllc_batch(int n, const double* WL, const double* WR, const double* pL,
+                             const double* pR, const double* gL, const double* gR,
+                             const double* du, const double* dv, const double* dw,
+                             double* flux, double* vface, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(hllc_batch_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, WL, WR,
+                       pL, pR, gL, gR, du, dv, dw, flux, vface);
+    return hipGetLastError();
+}
+
+hipError_t launch_hlld_batch(int n, const double* WL, const double* WR, const double* pL,
+                             const double* pR, const double* gL, const double* gR,
+                             const double* bn, double* flux, double* vface, double* bv1,
+                             double* bv2, int* status, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(hlld_batch_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, WL, WR,
+                       pL, pR, gL, gR, bn, flux, vface, bv1, bv2, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_emf_batch(int n, const double* ef, const double* ec, const double* vn,
+                            double* emf, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(emf_batch_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, ef, ec, vn,
+                       emf);
+    return hipGetLastError();
+}
+
+}  // namespace fx

@@ -88,7 +88,7 @@ def _lowpass_gaussian(rng, n, kmax):
     kz = np.fft.rfftfreq(n[2]) * n[2]
     k2 = kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2
     f[k2 > kmax * kmax] = 0.0
-    return np.fft.irfftn(f, s=n)
+    return np.fft.irfftn(f, s=n, axes=(0, 1, 2))
 
 
 def turbulent_box(n=(256, 256, 256), seed=1, gamma=5.0 / 3.0, rho0=1.0, p=1.0,
@@ -163,7 +163,7 @@ def random_smooth(n=(32, 32, 32), seed=0, bc=(PERIODIC,) * 6, gamma=1.4, mhd=Fal
             k = [np.fft.fftfreq(s[0]) * s[0], np.fft.fftfreq(s[1]) * s[1], np.fft.rfftfreq(s[2]) * s[2]]
             k2 = k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2
             f[k2 > 9] = 0.0
-            a = np.fft.irfftn(f, s=s)
+            a = np.fft.irfftn(f, s=s, axes=(0, 1, 2))
             A.append(bamp * a / np.std(a) * dx[0] * 4)
         Ax, Ay, Az = A
         # B = curl A on faces (exactly divergence-free in the discrete sense)

@@ -1,0 +1,84 @@
+"""Per-face HLLC/HLLD and per-edge EMF entry points (C ABI, device arrays)
+against the reference outputs in tests/golden/ -- bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _dev(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def _soa(a):
+    return _dev(np.ascontiguousarray(np.asarray(a).T))
+
+
+def test_hllc_batch(gpu):
+    import torch
+    d = np.load(os.path.join(GOLD, "hllc_vectors.npz"))
+    n = d["WL"].shape[0]
+    args = [_soa(d["WL"]), _soa(d["WR"])] + [_dev(d[k]) for k in ("pL", "pR", "gL", "gR", "du", "dv", "dw")]
+    flux = torch.empty((6, n), dtype=torch.float64, device="cuda")
+    vf = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    rc = gpu.fx_riemann_hllc_batch(n, *[a.data_ptr() for a in args], flux.data_ptr(), vf.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(flux.cpu().numpy().T, d["flux"])
+    assert np.array_equal(vf.cpu().numpy().T, d["v_face"])
+
+
+def test_hlld_batch(gpu):
+    import torch
+    d = np.load(os.path.join(GOLD, "hlld_vectors.npz"))
+    n = d["WL"].shape[0]
+    args = [_soa(d["WL"]), _soa(d["WR"])] + [_dev(d[k]) for k in ("pL", "pR", "gL", "gR", "bn")]
+    flux = torch.empty((9, n), dtype=torch.float64, device="cuda")
+    vf = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    b1 = torch.empty(n, dtype=torch.float64, device="cuda")
+    b2 = torch.empty(n, dtype=torch.float64, device="cuda")
+    st = torch.empty(n, dtype=torch.int32, device="cuda")
+    rc = gpu.fx_riemann_hlld_batch(n, *[a.data_ptr() for a in args], flux.data_ptr(), vf.data_ptr(),
+                                   b1.data_ptr(), b2.data_ptr(), st.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert int(st.abs().sum()) == 0
+    assert np.array_equal(flux.cpu().numpy().T, d["flux"])
+    assert np.array_equal(vf.cpu().numpy().T, d["v_face"])
+    assert np.array_equal(np.stack([b1.cpu().numpy(), b2.cpu().numpy()], 1), d["bv"])
+
+
+def test_hlld_batch_reports_bad_state(gpu):
+    import torch
+    d = np.load(os.path.join(GOLD, "hlld_vectors.npz"))
+    WL = d["WL"][:4].copy()
+    WL[1, 0] = -1.0            # non-positive density -> reference abort site
+    pL = d["pL"][:4].copy()
+    pL[2] = 0.0                # non-positive pressure
+    args = [_soa(WL), _soa(d["WR"][:4])] + [_dev(x) for x in (pL, d["pR"][:4], d["gL"][:4], d["gR"][:4], d["bn"][:4])]
+    outs = [torch.zeros((9, 4), dtype=torch.float64, device="cuda"),
+            torch.zeros((3, 4), dtype=torch.float64, device="cuda"),
+            torch.zeros(4, dtype=torch.float64, device="cuda"),
+            torch.zeros(4, dtype=torch.float64, device="cuda")]
+    st = torch.zeros(4, dtype=torch.int32, device="cuda")
+    rc = gpu.fx_riemann_hlld_batch(4, *[a.data_ptr() for a in args], *[o.data_ptr() for o in outs],
+                                   st.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert st.cpu().tolist() == [0, 11, 12, 0]
+
+
+def test_emf_batch(gpu):
+    import torch
+    d = np.load(os.path.join(GOLD, "emf_vectors.npz"))
+    n = d["emf"].shape[0]
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    rc = gpu.fx_emf_upwind_batch(n, _soa(d["emf_face"]).data_ptr(), _soa(d["emf_cell"]).data_ptr(),
+                                 _soa(d["v_norm"]).data_ptr(), out.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), d["emf"])

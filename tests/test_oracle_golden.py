@@ -1,0 +1,84 @@
+"""The C restatement against the committed golden fixtures (reference outputs).
+
+Fixtures come from tests/golden/make_golden.py (serial reference build): the
+reference's own per-face HLLC/HLLD and per-edge EMF functions on random
+states, and SHA-256 digests of whole RK2 trajectories.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import sha
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+_dp = C.POINTER(C.c_double)
+
+
+def _p(a):
+    return np.ascontiguousarray(a).ctypes.data_as(_dp)
+
+
+def test_hllc_vectors():
+    d = np.load(os.path.join(GOLD, "hllc_vectors.npz"))
+    lib = O.oracle_lib()
+    for i in range(d["WL"].shape[0]):
+        f = np.zeros(6)
+        v = np.zeros(3)
+        lib.fxo_hllc(_p(d["WL"][i]), _p(d["WR"][i]), d["pL"][i], d["pR"][i], d["gL"][i],
+                     d["gR"][i], _p(f), _p(v), d["du"][i], d["dv"][i], d["dw"][i])
+        assert np.array_equal(f, d["flux"][i]), i
+        assert np.array_equal(v, d["v_face"][i]), i
+
+
+def test_hlld_vectors():
+    d = np.load(os.path.join(GOLD, "hlld_vectors.npz"))
+    lib = O.oracle_lib()
+    for i in range(d["WL"].shape[0]):
+        f = np.zeros(9)
+        v = np.zeros(3)
+        b1 = C.c_double()
+        b2 = C.c_double()
+        rc = lib.fxo_hlld(_p(d["WL"][i]), _p(d["WR"][i]), d["pL"][i], d["pR"][i], d["gL"][i],
+                          d["gR"][i], d["bn"][i], _p(f), _p(v), C.byref(b1), C.byref(b2))
+        assert rc == 0
+        assert np.array_equal(f, d["flux"][i]), i
+        assert np.array_equal(v, d["v_face"][i]), i
+        assert (b1.value, b2.value) == tuple(d["bv"][i]), i
+
+
+def test_emf_vectors():
+    d = np.load(os.path.join(GOLD, "emf_vectors.npz"))
+    lib = O.oracle_lib()
+    for i in range(d["emf"].shape[0]):
+        out = C.c_double()
+        rc = lib.fxo_emf_upwind(_p(d["emf_face"][i]), _p(d["emf_cell"][i]), _p(d["v_norm"][i]),
+                                C.byref(out))
+        assert rc == 0 and out.value == d["emf"][i], i
+
+
+with open(os.path.join(GOLD, "trajectories.json")) as _f:
+    TRAJ = json.load(_f)
+
+import sys  # noqa: E402
+sys.path.insert(0, GOLD)
+import make_golden as MG  # noqa: E402
+
+
+@pytest.mark.parametrize("name", sorted(TRAJ))
+def test_trajectory(name):
+    rec = TRAJ[name]
+    pr, steps = MG.make_problem(name)
+    assert sha(pr.W) == rec["input_sha"], "initial-condition generator changed"
+    sim = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma, mhd=pr.mhd, eta=pr.eta)
+    sim.set_state(pr.W, pr.bf)
+    for s in range(steps):
+        dt = sim.step(pr.cfl)
+        assert dt == rec["dt"][s], (s, dt, rec["dt"][s])
+        assert sha(sim.W0) == rec["W_sha"][s], f"step {s}"
+        if pr.mhd:
+            assert [sha(b) for b in sim.bf0] == rec["bf_sha"][s]
