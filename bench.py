@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="cells per axis (multiple of 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="first run a plain device copy of this many GiB (3 times): the "
+                         "calibration kernel for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -93,6 +96,14 @@ def main():
     n = args.n
     pr = P.turbulent_box(n=(n, n, n), seed=1)
     torch.cuda.set_device(0)
+    if args.copy_gib > 0:
+        nelem = int(args.copy_gib * (1 << 30)) // 8
+        src = torch.ones(nelem, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     mesh = GpuMesh.from_problem(pr)
     for _ in range(args.warmup):
         mesh.step_async(pr.cfl)

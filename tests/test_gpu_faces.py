@@ -77,8 +77,8 @@ def test_emf_batch(gpu):
     d = np.load(os.path.join(GOLD, "emf_vectors.npz"))
     n = d["emf"].shape[0]
     out = torch.empty(n, dtype=torch.float64, device="cuda")
-    rc = gpu.fx_emf_upwind_batch(n, _soa(d["emf_face"]).data_ptr(), _soa(d["emf_cell"]).data_ptr(),
-                                 _soa(d["v_norm"]).data_ptr(), out.data_ptr(), None)
+    ins = [_soa(d["emf_face"]), _soa(d["emf_cell"]), _soa(d["v_norm"])]   # keep alive
+    rc = gpu.fx_emf_upwind_batch(n, *[a.data_ptr() for a in ins], out.data_ptr(), None)
     assert rc == 0
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), d["emf"])
