@@ -114,6 +114,8 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
 {
     __shared__ double pl[2][NQ][PC];
     __shared__ double v1p[PC];
+    __shared__ double exy[6][(BS + 1) * BS];    // y-face fluxes of plane i
+    __shared__ double exz[6][BS * (BS + 1)];    // z-face fluxes of plane i
 
     const Geom& g = a.g;
     const int t = threadIdx.x;
@@ -225,26 +227,23 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
     }
 
     const double a1 = g.area[0], a2 = g.area[1], a3 = g.area[2], vol = g.vol;
+    constexpr int EX = (BS + 1) * BS;        // 272 faces per direction and plane
 
     for (int i = 0; i < BS; ++i) {
         const int A = i & 1;
         const int B = A ^ 1;
-        double w0[6];
-        if (STAGE == 2) {
-#pragma unroll
-            for (int q = 0; q < 6; ++q) w0[q] = a.W0[q * vs + own_at(i)];
-        }
 
         // ---- faces: 0 = own y-face (low side), 1 = own z-face (low side),
-        //      2 = extra high-side face (one wave), 3 = x-face i+1 ------------
-        double Fy[6], Fz[6], Fe[6], Fxn[6];
+        //      2 = extra high-side face (one wave), 3 = x-face i+1.  y/z fluxes
+        //      go straight to the LDS exchange buffers. ----------------------
+        double Fxn[6];
         const int xw = i & 3;                  // wave solving the 32 extra faces
         const int lane = t & 63;
         const bool extra = (t >> 6) == xw && lane < 32;
         for (int f = 0; f < 4; ++f) {
             if (f == 2 && !extra) continue;
             double WL[6], WR[6], pL, pR, du, dv, dw;
-            int dir;
+            int dir, fj = j, fk = k;
             if (f == 3) {
                 // x-face between own cells i (cm) and i+1 (c0); cell i+2 is pf
                 double cp[NQ];
@@ -272,7 +271,6 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
                           dmin(B3[c] - B3[c - 1], B3[c + 1] - B3[c]));
                 dir = 0;
             } else {
-                int fj = j, fk = k;
                 if (f == 0) dir = 1;
                 else if (f == 1) dir = 2;
                 else if (lane < 16) { dir = 1; fj = BS; fk = lane; }
@@ -305,21 +303,15 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
                 const double* Xm = v1p;
                 const double* Xc = pl[A][1];
                 const double* Xp = pl[B][1];
-                if (dir == 1) {
-                    // dv: v3 along z (in plane), dw: v1 along x (planes i-1, i+1)
-                    const double* V3 = pl[A][3];
-                    dv = dmin(dmin(V3[cl] - V3[cl - T], V3[cl + T] - V3[cl]),
-                              dmin(V3[c] - V3[c - T], V3[c + T] - V3[c]));
-                    dw = dmin(dmin(Xc[cl] - Xm[cl], Xp[cl] - Xc[cl]),
-                              dmin(Xc[c] - Xm[c], Xp[c] - Xc[c]));
-                } else {
-                    // dv: v1 along x, dw: v2 along y (in plane)
-                    const double* V2 = pl[A][2];
-                    dv = dmin(dmin(Xc[cl] - Xm[cl], Xp[cl] - Xc[cl]),
-                              dmin(Xc[c] - Xm[c], Xp[c] - Xc[c]));
-                    dw = dmin(dmin(V2[cl] - V2[cl - T], V2[cl + T] - V2[cl]),
-                              dmin(V2[c] - V2[c - T], V2[c + T] - V2[c]));
-                }
+                const double* Vt = pl[A][dir == 1 ? 3 : 2];   // in-plane transverse velocity
+                double din = dmin(dmin(Vt[cl] - Vt[cl - T], Vt[cl + T] - Vt[cl]),
+                                  dmin(Vt[c] - Vt[c - T], Vt[c + T] - Vt[c]));
+                double dx1 = dmin(dmin(Xc[cl] - Xm[cl], Xp[cl] - Xc[cl]),
+                                  dmin(Xc[c] - Xm[c], Xp[c] - Xc[c]));
+                // y-face: dv = v3 along z (in plane), dw = v1 along x;
+                // z-face: dv = v1 along x, dw = v2 along y (in plane)
+                dv = dir == 1 ? din : dx1;
+                dw = dir == 1 ? dx1 : din;
             }
             double Fl[6], vf[3];
             hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fl, vf);
@@ -331,67 +323,43 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
             G[3] = dir == 0 ? Fl[3] : (dir == 1 ? Fl[2] : Fl[1]);
             G[4] = Fl[4];
             G[5] = Fl[5];
-            if (f == 0) {
-#pragma unroll
-                for (int v = 0; v < 6; ++v) Fy[v] = G[v];
-            } else if (f == 1) {
-#pragma unroll
-                for (int v = 0; v < 6; ++v) Fz[v] = G[v];
-            } else if (f == 2) {
-#pragma unroll
-                for (int v = 0; v < 6; ++v) Fe[v] = G[v];
-            } else {
+            if (f == 3) {
 #pragma unroll
                 for (int v = 0; v < 6; ++v) Fxn[v] = G[v];
+            } else if (dir == 1) {
+#pragma unroll
+                for (int v = 0; v < 6; ++v) exy[v][fj * BS + fk] = G[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 6; ++v) exz[v][fj * (BS + 1) + fk] = G[v];
             }
         }
-        __syncthreads();   // all reads of plane i's LDS data and of v1p are done
+        __syncthreads();   // fluxes written; all reads of plane i and v1p done
 
-        // ---- divergence: exchange y then z fluxes through the dead part of plane i
-        double* ex = &pl[A][2][0];              // 5*400 doubles >= 6*272
-        constexpr int EX = (BS + 1) * BS;        // 272
-#pragma unroll
-        for (int v = 0; v < 6; ++v) ex[v * EX + j * BS + k] = Fy[v];
-        if (extra && lane < 16) {
-#pragma unroll
-            for (int v = 0; v < 6; ++v) ex[v * EX + BS * BS + lane] = Fe[v];
-        }
-        v1p[oidx] = pl[A][1][oidx];
-        if (has_halo) v1p[hidx] = pl[A][1][hidx];
-        __syncthreads();
-        double part[6];
-#pragma unroll
-        for (int v = 0; v < 6; ++v) {
-            double fu = ex[v * EX + (j + 1) * BS + k];
-            part[v] = a1 * (Fxn[v] - Fx[v]) + a2 * (fu - Fy[v]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int v = 0; v < 6; ++v) ex[v * EX + j * (BS + 1) + k] = Fz[v];
-        if (extra && lane >= 16) {
-#pragma unroll
-            for (int v = 0; v < 6; ++v) ex[v * EX + (lane - 16) * (BS + 1) + BS] = Fe[v];
-        }
-        __syncthreads();
+        // ---- divergence in the reference's x, y, z order ---------------------
         double fd[6];
 #pragma unroll
         for (int v = 0; v < 6; ++v) {
-            double fu = ex[v * EX + j * (BS + 1) + k + 1];
-            fd[v] = -(part[v] + a3 * (fu - Fz[v])) / vol;
+            fd[v] = -(a1 * (Fxn[v] - Fx[v]) + a2 * (exy[v][(j + 1) * BS + k] - exy[v][j * BS + k]) +
+                      a3 * (exz[v][j * (BS + 1) + k + 1] - exz[v][j * (BS + 1) + k])) / vol;
         }
-        __syncthreads();   // exchange reads done; slot A is free for plane i+2
-
+        v1p[oidx] = pl[A][1][oidx];
+        if (has_halo) v1p[hidx] = pl[A][1][hidx];
+        __syncthreads();   // v1 of plane i saved; slot A is free for plane i+2
         if (i < BS - 1) store_plane(pl[A], pf, gm1, oidx, hidx, has_halo);
 
         // ---- per-cell RK2 update -------------------------------------------
         {
+            const int64_t o = own_at(i);
             double u[6], u1[6], w[6];
             if (STAGE == 1) {
                 prim2cons6(cm, u);
 #pragma unroll
                 for (int v = 0; v < 6; ++v) u1[v] = u[v] + dt * (0.0 + fd[v]);
             } else {
-                double us[6];
+                double us[6], w0[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) w0[q] = a.W0[q * vs + o];
                 prim2cons6(w0, u);
                 prim2cons6(cm, us);
 #pragma unroll
@@ -401,7 +369,6 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
                 }
             }
             cons2prim6(u1, w);
-            const int64_t o = own_at(i);
 #pragma unroll
             for (int v = 0; v < 6; ++v) __builtin_nontemporal_store(w[v], &a.Wout[v * vs + o]);
         }
