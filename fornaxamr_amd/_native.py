@@ -30,7 +30,8 @@ class FxError(RuntimeError):
 
 
 def lib_path():
-    return _build.LIB
+    """The in-tree library, or the file named by FORNAX_GPU_LIB."""
+    return os.environ.get("FORNAX_GPU_LIB") or _build.LIB
 
 
 def load():

@@ -32,7 +32,8 @@ namespace {
 
 constexpr int PW = BS + 2 * NG;    // 20
 constexpr int PC = PW * PW;        // 400 cells per plane incl. halo
-constexpr int NQ = 7;              // rho v1 v2 v3 e ye p
+constexpr int NQ = 7;              // rho v1 v2 v3 e ye p (x-column states)
+constexpr int NV = 6;              // rho v1 v2 v3 e ye (LDS planes)
 constexpr int NHALO = PC - BS * BS; // 144
 constexpr int NT = 256;
 
@@ -53,16 +54,14 @@ __device__ __forceinline__ void load_plane(const double* __restrict__ W, int64_t
     }
 }
 
-__device__ __forceinline__ void store_plane(double (*S)[PC], const PlaneRegs& r, double gm1,
-                                            int oidx, int hidx, bool has_halo)
+__device__ __forceinline__ void store_plane(double (*S)[PC], const PlaneRegs& r, int oidx,
+                                            int hidx, bool has_halo)
 {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) S[q][oidx] = r.own[q];
-    S[6][oidx] = gm1 * r.own[0] * r.own[4];
+    for (int q = 0; q < NV; ++q) S[q][oidx] = r.own[q];
     if (has_halo) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) S[q][hidx] = r.halo[q];
-        S[6][hidx] = gm1 * r.halo[0] * r.halo[4];
+        for (int q = 0; q < NV; ++q) S[q][hidx] = r.halo[q];
     }
 }
 
@@ -112,15 +111,21 @@ __device__ __forceinline__ int block_of_workgroup(int wg, int nwg)
 template <int STAGE>
 __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
 {
-    __shared__ double pl[2][NQ][PC];
+    // planes i and i+1 (rho v1 v2 v3 e ye; pressure is formed from rho*e on
+    // use), v1 of plane i-1, and the face fluxes that cross a wave boundary
+    __shared__ double pl[2][NV][PC];
     __shared__ double v1p[PC];
-    __shared__ double exy[6][(BS + 1) * BS];    // y-face fluxes of plane i
-    __shared__ double exz[6][BS * (BS + 1)];    // z-face fluxes of plane i
+    __shared__ double exup[6][4][BS];   // Fy(4(w+1), k) for the last row of wave w
+    __shared__ double ez16[6][BS];      // Fz(j, 16): the block's high-side z faces
+    __shared__ double hy[NQ][288];      // MC half-slopes along y, cells j=-1..16, k=0..15
+    __shared__ double hz[NQ][288];      // MC half-slopes along z, cells j=0..15, k=-1..16
 
     const Geom& g = a.g;
     const int t = threadIdx.x;
     const int j = t >> 4;
     const int k = t & 15;
+    const int lane = t & 63;
+    const int wave = t >> 6;
     const int b = block_of_workgroup(blockIdx.x, gridDim.x);
     const int bz = b % g.nbz;
     const int by = (b / g.nbz) % g.nby;
@@ -152,112 +157,151 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
     auto own_at = [&](int p) { return col_own + (int64_t)p * g.sx; };
     auto halo_at = [&](int p) { return col_halo + (int64_t)p * g.sx; };
 
-    // ---- prologue: planes -2..2 -----------------------------------------
-    double cm[NQ], c0[NQ];     // column cells i and i+1 (q incl. p)
-    double Ls[NQ];             // left state of x-face i+1
+    double Ls[NQ];             // left state of x-face i+1 (rho..ye, p)
     double Fx[6];              // flux through x-face i
     PlaneRegs pf;              // prefetched plane i+2
 
+    // ---- prologue: planes -2..2 -----------------------------------------
     {
-        double cmm[NQ];
+        double cmm[NQ], cm[NQ], c0[NQ], c1[NQ];
         PlaneRegs r;
-        // plane -1 -> slot 1 (full) and v1p; plane 0 -> slot 0
         load_plane(a.Win, vs, own_at(-1), halo_at(-1), has_halo, r);
-        store_plane(pl[1], r, gm1, oidx, hidx, has_halo);
+        store_plane(pl[1], r, oidx, hidx, has_halo);        // plane -1 (temporary)
         v1p[oidx] = r.own[1];
         if (has_halo) v1p[hidx] = r.halo[1];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cm[q] = r.own[q];   // cm temporarily = cell -1
-        cm[6] = gm1 * cm[0] * cm[4];
+        for (int q = 0; q < NV; ++q) cm[q] = r.own[q];
         load_plane(a.Win, vs, own_at(0), halo_at(0), has_halo, r);
-        store_plane(pl[0], r, gm1, oidx, hidx, has_halo);
+        store_plane(pl[0], r, oidx, hidx, has_halo);        // plane 0
 #pragma unroll
-        for (int q = 0; q < 6; ++q) c0[q] = r.own[q];   // c0 temporarily = cell 0
-        c0[6] = gm1 * c0[0] * c0[4];
-        double c1[NQ];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
+        for (int q = 0; q < NV; ++q) {
+            c0[q] = r.own[q];
             cmm[q] = a.Win[q * vs + own_at(-2)];
             c1[q] = a.Win[q * vs + own_at(1)];
         }
         cmm[6] = gm1 * cmm[0] * cmm[4];
+        cm[6] = gm1 * cm[0] * cm[4];
+        c0[6] = gm1 * c0[0] * c0[4];
         c1[6] = gm1 * c1[0] * c1[4];
         __syncthreads();
 
         // x-face 0: L from cell -1, R from cell 0
         double WL[6], WR[6], pL, pR;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            WL[q] = cm[q] + mc_half(cmm[q], cm[q], c0[q]);
+        for (int q = 0; q < NQ; ++q) {
+            double hl = mc_half(cmm[q], cm[q], c0[q]);
             double h0 = mc_half(cm[q], c0[q], c1[q]);
-            WR[q] = c0[q] - h0;
+            if (q < 6) { WL[q] = cm[q] + hl; WR[q] = c0[q] - h0; }
+            else { pL = cm[q] + hl; pR = c0[q] - h0; }
             Ls[q] = c0[q] + h0;
         }
-        pL = cm[6] + mc_half(cmm[6], cm[6], c0[6]);
-        {
-            double h0 = mc_half(cm[6], c0[6], c1[6]);
-            pR = c0[6] - h0;
-            Ls[6] = c0[6] + h0;
-        }
-        // deltas: L cell in plane -1 (slot 1), R cell in plane 0 (slot 0)
         const double* A2 = pl[1][2];
         const double* A3 = pl[1][3];
         const double* B2 = pl[0][2];
         const double* B3 = pl[0][3];
         const int c = oidx;
         double du = c0[1] - cm[1];
-        double dv = dmin(dmin(A2[c] - A2[c - PW], A2[c + PW] - A2[c]),
-                         dmin(B2[c] - B2[c - PW], B2[c + PW] - B2[c]));
-        double dw = dmin(dmin(A3[c] - A3[c - 1], A3[c + 1] - A3[c]),
-                         dmin(B3[c] - B3[c - 1], B3[c + 1] - B3[c]));
+        double dv = dmin_delta(dmin_delta(A2[c] - A2[c - PW], A2[c + PW] - A2[c]),
+                         dmin_delta(B2[c] - B2[c - PW], B2[c + PW] - B2[c]));
+        double dw = dmin_delta(dmin_delta(A3[c] - A3[c - 1], A3[c + 1] - A3[c]),
+                         dmin_delta(B3[c] - B3[c - 1], B3[c + 1] - B3[c]));
         double vf[3];
         hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fx, vf);
-
-        // window for step 0: cm = cell 0, c0 = cell 1
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            cm[q] = c0[q];
-            c0[q] = c1[q];
-        }
         __syncthreads();
         load_plane(a.Win, vs, own_at(1), halo_at(1), has_halo, r);
-        store_plane(pl[1], r, gm1, oidx, hidx, has_halo);
+        store_plane(pl[1], r, oidx, hidx, has_halo);        // plane 1
         load_plane(a.Win, vs, own_at(2), halo_at(2), has_halo, pf);
         __syncthreads();
     }
 
     const double a1 = g.area[0], a2 = g.area[1], a3 = g.area[2], vol = g.vol;
-    constexpr int EX = (BS + 1) * BS;        // 272 faces per direction and plane
+    const double inv_vol = g.inv_vol;
+    const bool vol_pow2 = g.vol_pow2 != 0;
 
     for (int i = 0; i < BS; ++i) {
         const int A = i & 1;
         const int B = A ^ 1;
 
+        // ---- MC half-slopes of plane i along y and z, once per cell ---------
+        // (the reference sweeps each variable once per direction and emits both
+        // faces of a cell from one slope, src/prj_flux.c:373-445)
+        {
+            const double* R = pl[A][0];
+            const double* E = pl[A][4];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                double qm_y, qp_y, qm_z, qp_z, q0;
+                if (q < NV) {
+                    const double* P = pl[A][q];
+                    q0 = P[oidx];
+                    qm_y = P[oidx - PW]; qp_y = P[oidx + PW];
+                    qm_z = P[oidx - 1]; qp_z = P[oidx + 1];
+                } else {
+                    q0 = gm1 * R[oidx] * E[oidx];
+                    qm_y = gm1 * R[oidx - PW] * E[oidx - PW];
+                    qp_y = gm1 * R[oidx + PW] * E[oidx + PW];
+                    qm_z = gm1 * R[oidx - 1] * E[oidx - 1];
+                    qp_z = gm1 * R[oidx + 1] * E[oidx + 1];
+                }
+                hy[q][(j + 1) * BS + k] = mc_half(qm_y, q0, qp_y);
+                hz[q][j * (BS + 2) + k + 1] = mc_half(qm_z, q0, qp_z);
+            }
+            // the 64 halo cells whose slopes the block's boundary faces need:
+            // y: rows -1 and 16; z: columns -1 and 16 (one wave, one cell per lane)
+            if (wave == ((i + 1) & 3)) {
+                const int side = lane >> 4;          // 0,1: y rows -1,16; 2,3: z cols -1,16
+                const int m = lane & 15;
+                int cj, ck, S;
+                if (side < 2) { cj = side == 0 ? -1 : BS; ck = m; S = PW; }
+                else { cj = m; ck = side == 2 ? -1 : BS; S = 1; }
+                const int c = (cj + NG) * PW + (ck + NG);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    double a0, a1v, a2v;
+                    if (q < NV) {
+                        const double* P = pl[A][q];
+                        a0 = P[c - S]; a1v = P[c]; a2v = P[c + S];
+                    } else {
+                        a0 = gm1 * R[c - S] * E[c - S];
+                        a1v = gm1 * R[c] * E[c];
+                        a2v = gm1 * R[c + S] * E[c + S];
+                    }
+                    double h = mc_half(a0, a1v, a2v);
+                    if (side < 2) hy[q][(cj + 1) * BS + ck] = h;
+                    else hz[q][cj * (BS + 2) + ck + 1] = h;
+                }
+            }
+        }
+        __syncthreads();
+
         // ---- faces: 0 = own y-face (low side), 1 = own z-face (low side),
-        //      2 = extra high-side face (one wave), 3 = x-face i+1.  y/z fluxes
-        //      go straight to the LDS exchange buffers. ----------------------
-        double Fxn[6];
-        const int xw = i & 3;                  // wave solving the 32 extra faces
-        const int lane = t & 63;
-        const bool extra = (t >> 6) == xw && lane < 32;
+        //      2 = extra high-side face (half of one wave), 3 = x-face i+1 ----
+        double Fy[6], Fz[6], Fxn[6];
+        const int xw = i & 3;
+        const bool extra = wave == xw && lane < 32;
         for (int f = 0; f < 4; ++f) {
             if (f == 2 && !extra) continue;
             double WL[6], WR[6], pL, pR, du, dv, dw;
             int dir, fj = j, fk = k;
             if (f == 3) {
-                // x-face between own cells i (cm) and i+1 (c0); cell i+2 is pf
-                double cp[NQ];
+                // x-face between own cells i (slot A) and i+1 (slot B); cell i+2 is pf
+                double cm[NQ], c0[NQ], cp[NQ];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) cp[q] = pf.own[q];
+                for (int q = 0; q < NV; ++q) {
+                    cm[q] = pl[A][q][oidx];
+                    c0[q] = pl[B][q][oidx];
+                    cp[q] = pf.own[q];
+                }
+                cm[6] = gm1 * cm[0] * cm[4];
+                c0[6] = gm1 * c0[0] * c0[4];
                 cp[6] = gm1 * cp[0] * cp[4];
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
                     double h = mc_half(cm[q], c0[q], cp[q]);
                     double r = c0[q] - h;
-                    double l = c0[q] + h;
                     if (q < 6) { WL[q] = Ls[q]; WR[q] = r; }
                     else { pL = Ls[6]; pR = r; }
-                    Ls[q] = l;
+                    Ls[q] = c0[q] + h;
                 }
                 const double* A2 = pl[A][2];
                 const double* A3 = pl[A][3];
@@ -265,10 +309,10 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
                 const double* B3 = pl[B][3];
                 const int c = oidx;
                 du = c0[1] - cm[1];
-                dv = dmin(dmin(A2[c] - A2[c - PW], A2[c + PW] - A2[c]),
-                          dmin(B2[c] - B2[c - PW], B2[c + PW] - B2[c]));
-                dw = dmin(dmin(A3[c] - A3[c - 1], A3[c + 1] - A3[c]),
-                          dmin(B3[c] - B3[c - 1], B3[c + 1] - B3[c]));
+                dv = dmin_delta(dmin_delta(A2[c] - A2[c - PW], A2[c + PW] - A2[c]),
+                                dmin_delta(B2[c] - B2[c - PW], B2[c + PW] - B2[c]));
+                dw = dmin_delta(dmin_delta(A3[c] - A3[c - 1], A3[c + 1] - A3[c]),
+                                dmin_delta(B3[c] - B3[c - 1], B3[c + 1] - B3[c]));
                 dir = 0;
             } else {
                 if (f == 0) dir = 1;
@@ -279,23 +323,26 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
                 const int S = dir == 1 ? PW : 1;             // stride along the face normal
                 const int T = dir == 1 ? 1 : PW;             // in-plane transverse stride
                 const int cl = c - S;
-                // local order: rho, vn, vt1, vt2, e, ye
+                // slope-plane indices of the left and right cells
+                const int hr = dir == 1 ? (fj + 1) * BS + fk : fj * (BS + 2) + fk + 1;
+                const int hl = dir == 1 ? hr - BS : hr - 1;
+                const double (*H)[288] = dir == 1 ? hy : hz;
                 const int qn = dir == 1 ? 2 : 3;
                 const int qt1 = dir == 1 ? 3 : 1;
                 const int qt2 = dir == 1 ? 1 : 2;
                 const int qmap[6] = {0, qn, qt1, qt2, 4, 5};
 #pragma unroll
                 for (int lv = 0; lv < 6; ++lv) {
-                    const double* P = pl[A][qmap[lv]];
-                    double q0 = P[c - 2 * S], q1 = P[cl], q2 = P[c], q3 = P[c + S];
-                    WL[lv] = q1 + mc_half(q0, q1, q2);
-                    WR[lv] = q2 - mc_half(q1, q2, q3);
+                    const int q = qmap[lv];
+                    const double* P = pl[A][q];
+                    WL[lv] = P[cl] + H[q][hl];
+                    WR[lv] = P[c] - H[q][hr];
                 }
                 {
-                    const double* P = pl[A][6];
-                    double q0 = P[c - 2 * S], q1 = P[cl], q2 = P[c], q3 = P[c + S];
-                    pL = q1 + mc_half(q0, q1, q2);
-                    pR = q2 - mc_half(q1, q2, q3);
+                    const double* R = pl[A][0];
+                    const double* E = pl[A][4];
+                    pL = gm1 * R[cl] * E[cl] + H[6][hl];
+                    pR = gm1 * R[c] * E[c] - H[6][hr];
                 }
                 // velocity deltas (src/prj_flux.c:1967-2036)
                 const double* Vn = pl[A][qn];
@@ -303,11 +350,11 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
                 const double* Xm = v1p;
                 const double* Xc = pl[A][1];
                 const double* Xp = pl[B][1];
-                const double* Vt = pl[A][dir == 1 ? 3 : 2];   // in-plane transverse velocity
-                double din = dmin(dmin(Vt[cl] - Vt[cl - T], Vt[cl + T] - Vt[cl]),
-                                  dmin(Vt[c] - Vt[c - T], Vt[c + T] - Vt[c]));
-                double dx1 = dmin(dmin(Xc[cl] - Xm[cl], Xp[cl] - Xc[cl]),
-                                  dmin(Xc[c] - Xm[c], Xp[c] - Xc[c]));
+                const double* Vt = pl[A][dir == 1 ? 3 : 2];
+                double din = dmin_delta(dmin_delta(Vt[cl] - Vt[cl - T], Vt[cl + T] - Vt[cl]),
+                                        dmin_delta(Vt[c] - Vt[c - T], Vt[c + T] - Vt[c]));
+                double dx1 = dmin_delta(dmin_delta(Xc[cl] - Xm[cl], Xp[cl] - Xc[cl]),
+                                        dmin_delta(Xc[c] - Xm[c], Xp[c] - Xc[c]));
                 // y-face: dv = v3 along z (in plane), dw = v1 along x;
                 // z-face: dv = v1 along x, dw = v2 along y (in plane)
                 dv = dir == 1 ? din : dx1;
@@ -323,37 +370,65 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
             G[3] = dir == 0 ? Fl[3] : (dir == 1 ? Fl[2] : Fl[1]);
             G[4] = Fl[4];
             G[5] = Fl[5];
-            if (f == 3) {
+            if (f == 0) {
 #pragma unroll
-                for (int v = 0; v < 6; ++v) Fxn[v] = G[v];
-            } else if (dir == 1) {
+                for (int v = 0; v < 6; ++v) Fy[v] = G[v];
+                // first row of waves 1..3 is the high face of the previous wave's last row
+                if (wave > 0 && lane < BS) {
 #pragma unroll
-                for (int v = 0; v < 6; ++v) exy[v][fj * BS + fk] = G[v];
+                    for (int v = 0; v < 6; ++v) exup[v][wave - 1][k] = G[v];
+                }
+            } else if (f == 1) {
+#pragma unroll
+                for (int v = 0; v < 6; ++v) Fz[v] = G[v];
+            } else if (f == 2) {
+                if (dir == 1) {
+#pragma unroll
+                    for (int v = 0; v < 6; ++v) exup[v][3][fk] = G[v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 6; ++v) ez16[v][fj] = G[v];
+                }
             } else {
 #pragma unroll
-                for (int v = 0; v < 6; ++v) exz[v][fj * (BS + 1) + fk] = G[v];
+                for (int v = 0; v < 6; ++v) Fxn[v] = G[v];
             }
         }
-        __syncthreads();   // fluxes written; all reads of plane i and v1p done
+        // own cell of plane i for the update (slot A is overwritten below)
+        double wi[6];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) wi[q] = pl[A][q][oidx];
+        __syncthreads();   // cross-wave fluxes written; plane i, v1p, slope reads done
 
         // ---- divergence in the reference's x, y, z order ---------------------
-        double fd[6];
+        double fs[6];
 #pragma unroll
         for (int v = 0; v < 6; ++v) {
-            fd[v] = -(a1 * (Fxn[v] - Fx[v]) + a2 * (exy[v][(j + 1) * BS + k] - exy[v][j * BS + k]) +
-                      a3 * (exz[v][j * (BS + 1) + k + 1] - exz[v][j * (BS + 1) + k])) / vol;
+            double fyu = __shfl_down(Fy[v], BS);
+            double fzu = __shfl_down(Fz[v], 1);
+            if (lane >= 48) fyu = exup[v][wave][k];
+            if (k == BS - 1) fzu = ez16[v][j];
+            fs[v] = -(a1 * (Fxn[v] - Fx[v]) + a2 * (fyu - Fy[v]) + a3 * (fzu - Fz[v]));
+        }
+        double fd[6];
+        if (vol_pow2) {
+#pragma unroll
+            for (int v = 0; v < 6; ++v) fd[v] = fs[v] * inv_vol;
+        } else {
+#pragma unroll
+            for (int v = 0; v < 6; ++v) fd[v] = fs[v] / vol;
         }
         v1p[oidx] = pl[A][1][oidx];
         if (has_halo) v1p[hidx] = pl[A][1][hidx];
-        __syncthreads();   // v1 of plane i saved; slot A is free for plane i+2
-        if (i < BS - 1) store_plane(pl[A], pf, gm1, oidx, hidx, has_halo);
+        __syncthreads();   // v1 of plane i saved, exchange reads done: slot A is free
+        if (i < BS - 1) store_plane(pl[A], pf, oidx, hidx, has_halo);
 
         // ---- per-cell RK2 update -------------------------------------------
         {
             const int64_t o = own_at(i);
             double u[6], u1[6], w[6];
             if (STAGE == 1) {
-                prim2cons6(cm, u);
+                prim2cons6(wi, u);
 #pragma unroll
                 for (int v = 0; v < 6; ++v) u1[v] = u[v] + dt * (0.0 + fd[v]);
             } else {
@@ -361,7 +436,7 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
 #pragma unroll
                 for (int q = 0; q < 6; ++q) w0[q] = a.W0[q * vs + o];
                 prim2cons6(w0, u);
-                prim2cons6(cm, us);
+                prim2cons6(wi, us);
 #pragma unroll
                 for (int v = 0; v < 6; ++v) {
                     double ut = us[v] + dt * (0.0 + fd[v]);
@@ -373,14 +448,8 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
             for (int v = 0; v < 6; ++v) __builtin_nontemporal_store(w[v], &a.Wout[v * vs + o]);
         }
 
-        // ---- advance the column window and prefetch plane i+3 ----------------
 #pragma unroll
         for (int q = 0; q < 6; ++q) Fx[q] = Fxn[q];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) cm[q] = c0[q];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) c0[q] = pf.own[q];
-        c0[6] = gm1 * c0[0] * c0[4];
         if (i + 3 <= BS + 1) load_plane(a.Win, vs, own_at(i + 3), halo_at(i + 3), has_halo, pf);
         __syncthreads();
     }

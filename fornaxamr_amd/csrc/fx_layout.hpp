@@ -12,6 +12,7 @@
 // (face x of Bx at X in [-2, NX+3)).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 
 namespace fx {
@@ -31,6 +32,8 @@ struct Geom {
     double dx[3];
     double area[3];
     double vol;
+    double inv_vol;             // 1/vol, exact when vol_pow2
+    int vol_pow2;               // vol is a power of two: x/vol == x*inv_vol bit for bit
     double gamma;
     int bc[6];                  // reference codes: 0 outflow, 1 reflect, 3 periodic
     int mhd;
@@ -64,6 +67,12 @@ inline Geom make_geom(int nx, int ny, int nz, const double* xlim, const int* bc,
     g.area[0] = g.dx[1] * g.dx[2];
     g.area[1] = g.dx[0] * g.dx[2];
     g.area[2] = g.dx[0] * g.dx[1];
+    {
+        int e = 0;
+        double m = std::frexp(g.vol, &e);   // vol = m * 2^e, m in [0.5, 1)
+        g.vol_pow2 = (m == 0.5) ? 1 : 0;
+        g.inv_vol = g.vol_pow2 ? std::ldexp(1.0, 1 - e) : 1.0 / g.vol;
+    }
     g.gamma = gamma;
     for (int i = 0; i < 6; ++i) g.bc[i] = bc[i];
     g.mhd = mhd;

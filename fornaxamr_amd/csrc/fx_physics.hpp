@@ -55,13 +55,20 @@ __device__ __forceinline__ double mc_half(double qm, double q0, double qp)
     return (sl * sr > 0.0) ? h : 0.0;
 }
 
-// Shock-detector pressure-jump limiter (src/prj_riemann.c:8-17).
+// Shock-detector pressure-jump limiter (src/prj_riemann.c:8-17).  The
+// velocity deltas enter only through -min(., 0) + cfmax with cfmax > 0, so a
+// min that differs from the reference's ternary only in the sign of a zero
+// result gives the same theta; fmin (one instruction) is used throughout the
+// delta chain.
 __device__ __forceinline__ double theta_limiter(double cfmax, double du, double dv, double dw)
 {
-    double t = (-dmin(du, 0.0) + cfmax) / (-dmin(dmin(dv, dw), 0.0) + cfmax);
+    double t = (-fmin(du, 0.0) + cfmax) / (-fmin(fmin(dv, dw), 0.0) + cfmax);
     t *= t;
-    return dmin(1.0, t * t);
+    return fmin(1.0, t * t);
 }
+
+// min used for the velocity deltas (see theta_limiter)
+__device__ __forceinline__ double dmin_delta(double a, double b) { return fmin(a, b); }
 
 // HLLC with the theta-limited contact speed (src/prj_riemann.c:1433-1599),
 // hydro build (6 variables).  W* are face states in the rotated frame.
