@@ -12,6 +12,15 @@ roofline (event-timed live on the mesh's stream) and a CPU baseline: the
 serial reference build (oracle/_ref) when present, else the C restatement,
 timed on this host's cores over a bounded sample (the same scheme on a 64^3
 periodic box, one process per core).
+
+The same line carries the batched-ensemble figure: 48 independent periodic
+turbulent boxes with edges drawn from {32,48,64,96,128} and gamma from
+[1.3,1.7] (fornaxamr_amd.problems.ragged_ensemble), advanced by one batched
+launch per stage captured in a hipGraph with the autotuned schedule, against
+one set of launches per instance, and the cost of the discrete adjoint (reverse
+sweep with recompute) relative to the forward on the 64^3 acoustic box.
+``--workload mhd`` prints the line of the MHD stage instead (magnetized Sedov,
+PLM(MC)+HLLD+upwind CT).
 """
 from __future__ import annotations
 
@@ -29,6 +38,93 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per zone-update: stage 1 reads W (6) + P, gamma (2) and
 # writes W (6); stage 2 also reads W^n (6)  ->  112 B and 160 B
 BYTES_STAGE = {1: 112.0, 2: 160.0}
+# MHD: + B (3 cell, 3 face) read and written, + W^n and Bf^n in stage 2
+BYTES_STAGE_MHD = {1: 208.0, 2: 304.0}
+METRIC = "FP64 zone-updates/s per RK stage (plain vs batched) + adjoint/forward cost ratio"
+
+
+def _timed_steps(step, sync, steps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    sync()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def run_ensemble(steps, warmup, count=48):
+    """Batched (autotuned, graph-captured) vs per-instance launches."""
+    from fornaxamr_amd import problems as P
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    probs = P.ragged_ensemble(count=count)
+    e = GpuEnsemble(probs)
+    cfl = 0.4
+
+    def measure(graph):
+        e.reset()
+        e.set_graph(graph)
+        for _ in range(warmup):
+            e.step_async(cfl)
+        e.sync()
+        return _timed_steps(lambda: e.step_async(cfl), e.sync, steps) / steps
+
+    e.set_schedule(0, 0, 1, 0)
+    t_inst = measure(False)
+    t_inst_graph = measure(True)
+    e.set_graph(True)
+    best = e.autotune(cfl, 1.2, reps=3)
+    t_batch = measure(True)
+    zu = e.ncells * 2
+    e.close()
+    sizes = sorted({p.n[0] for p in probs})
+    return {
+        "value": zu / t_batch, "unit": "zone-updates/s",
+        "per_instance_value": zu / t_inst,
+        "per_instance_graph_value": zu / t_inst_graph,
+        "speedup_vs_per_instance": t_inst / t_batch,
+        "ms_per_step": t_batch * 1000.0, "per_instance_ms_per_step": t_inst * 1000.0,
+        "schedule": {k: best[k] for k in ("mode", "persistent_wgs", "xcd", "table")},
+        "workload": (f"{count} periodic turbulent boxes, edges from {sizes}, gamma U[1.3,1.7], "
+                     f"{e.nblocks} blocks, {e.ncells} cells, hipGraph per step"),
+    }
+
+
+def run_adjoint(reps=3):
+    """Cost of the discrete adjoint (reverse sweep incl. recompute) against the
+    forward, on the acoustic box: 64^3, 40 RK2 steps, checkpoint every 8."""
+    import torch
+    from fornaxamr_amd import problems as P
+    from fornaxamr_amd.adjoint import AdjointSolver
+    pr = P.acoustic_box(n=(64, 64, 64))
+    dt = P.cfl_dt(pr.W, pr.gamma, [1.0 / 64] * 3, pr.cfl)
+    solver = AdjointSolver(pr.n, pr.xlim, pr.gamma, checkpoint_every=8)
+    W0 = torch.from_numpy(pr.W).cuda()
+    mask = torch.from_numpy(P.subbox_mask(pr.n, pr.xlim)).cuda()
+    nsteps = 40
+    WT = solver.forward(W0, nsteps, dt)
+    gWT = torch.zeros_like(WT)
+    gWT[0] = 2.0 * WT[0] * mask
+    solver.backward(gWT)
+    torch.cuda.synchronize()
+    tf = tb = 1e30
+    for _ in range(reps):
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record()
+        solver.forward(W0, nsteps, dt)
+        b.record()
+        solver.backward(gWT)
+        c.record()
+        torch.cuda.synchronize()
+        tf = min(tf, a.elapsed_time(b))
+        tb = min(tb, b.elapsed_time(c))
+    solver.close()
+    zu = pr.ncells * 2 * nsteps
+    return {"ratio": tb / tf, "forward_ms": tf, "backward_ms": tb,
+            "forward_zone_updates_per_s": zu / (tf / 1000.0),
+            "workload": "van Albada + HLLE RK2, 64^3 periodic acoustic box, 40 steps, "
+                        "checkpoint every 8 (reverse sweep recomputes each segment)"}
 
 
 def _cpu_worker(args):
@@ -70,31 +166,70 @@ def cpu_baseline(nsteps=6):
                        f"{'serial reference build' if kind == 'reference' else 'C restatement'}")}
 
 
+def run_mesh(pr, bytes_stage, steps, warmup):
+    """Steps of one device-resident mesh; stage kernels event-timed on its stream."""
+    from fornaxamr_amd.mesh import GpuMesh
+    mesh = GpuMesh.from_problem(pr)
+    for _ in range(warmup):
+        mesh.step_async(pr.cfl)
+    mesh.sync()
+    mesh.set_timing(True)
+    elapsed = _timed_steps(lambda: mesh.step_async(pr.cfl), mesh.sync, steps)
+    ms1, n1 = mesh.kernel_time(1)
+    ms2, n2 = mesh.kernel_time(2)
+    mesh.set_timing(False)
+    mesh.close()
+    ncells = pr.ncells
+    bytes_launch_avg = ncells * (bytes_stage[1] + bytes_stage[2]) / 2.0
+    avg_launch_s = (ms1 + ms2) / 1000.0 / (n1 + n2)
+    achieved = bytes_launch_avg / avg_launch_s / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_ms": avg_launch_s * 1000.0,
+            "stage_ms": [ms1 / max(n1, 1), ms2 / max(n2, 1)]}
+    return ncells * 2 * steps / elapsed, elapsed / steps * 1000.0, roof
+
+
+def _traffic(name):
+    """HBM bytes per launch from the committed counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=256, help="cells per axis (multiple of 16)")
+    ap.add_argument("--workload", choices=["hydro", "mhd"], default="hydro")
+    ap.add_argument("--n", type=int, default=None, help="cells per axis (multiple of 16); "
+                    "default 256 (hydro), 128 (mhd)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble figure")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost ratio")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the batched-ensemble figure (for profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="first run a plain device copy of this many GiB (3 times): the "
                          "calibration kernel for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(the stage shards by blocks; see DESIGN.md)", file=sys.stderr)
+              f"(the stage shards by blocks over GPUs; on one GPU every block is resident)",
+              file=sys.stderr)
         sys.exit(2)
+    mhd = args.workload == "mhd"
+    n = args.n or (128 if mhd else 256)
 
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None if (args.no_cpu or mhd or args.ensemble_only) else cpu_baseline()
 
-    import numpy as np  # noqa: F401
     import torch
     from fornaxamr_amd import problems as P
-    from fornaxamr_amd.mesh import GpuMesh
 
-    n = args.n
-    pr = P.turbulent_box(n=(n, n, n), seed=1)
     torch.cuda.set_device(0)
     if args.copy_gib > 0:
         nelem = int(args.copy_gib * (1 << 30)) // 8
@@ -104,52 +239,48 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    mesh = GpuMesh.from_problem(pr)
-    for _ in range(args.warmup):
-        mesh.step_async(pr.cfl)
-    mesh.sync()
 
-    mesh.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mesh.step_async(pr.cfl)
-    mesh.sync()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    if args.ensemble_only:
+        ens = run_ensemble(args.steps, args.warmup)
+        print(json.dumps({"ensemble": ens}), flush=True)
+        return
 
-    ms1, n1 = mesh.kernel_time(1)
-    ms2, n2 = mesh.kernel_time(2)
-    mesh.set_timing(False)
-    ncells = pr.ncells
-    zu = ncells * 2 * args.steps
-    value = zu / elapsed
-    bytes_launch_avg = ncells * (BYTES_STAGE[1] + BYTES_STAGE[2]) / 2.0
-    avg_launch_s = (ms1 + ms2) / 1000.0 / (n1 + n2)
-    achieved = bytes_launch_avg / avg_launch_s / 1e9
+    if mhd:
+        pr = P.magnetized_sedov(n=(n, n, n))
+        value, ms, roof = run_mesh(pr, BYTES_STAGE_MHD, args.steps, args.warmup)
+        roof["kernel"] = "mhd_flux_kernel + mhd_ghostface_kernel + mhd_ct_kernel"
+        roof["traffic"] = _traffic(f"mhd_{n}")
+        workload = (f"MHD PLM(MC)+HLLD+upwind CT RK2, {n}^3 magnetized Sedov (outflow), "
+                    f"{pr.nblocks} blocks of 16^3, CFL 0.4")
+    else:
+        pr = P.turbulent_box(n=(n, n, n), seed=1)
+        value, ms, roof = run_mesh(pr, BYTES_STAGE, args.steps, args.warmup)
+        roof["kernel"] = "hydro_stage_kernel<1|2>"
+        roof["traffic"] = _traffic(f"hydro_{n}")
+        workload = (f"hydro PLM(MC)+HLLC RK2, {n}^3 periodic turbulent box, "
+                    f"{pr.nblocks} blocks of 16^3, CFL 0.4")
     out = {
-        "metric": "FP64 zone-updates/s per RK stage",
+        "metric": METRIC,
         "value": value,
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1000.0,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"hydro PLM(MC)+HLLC RK2, {n}^3 periodic turbulent box, "
-                               f"{pr.nblocks} blocks of 16^3, CFL 0.4",
-                   "cells": ncells, "blocks": pr.nblocks, "stages_per_step": 2},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "hydro_stage_kernel<1|2>",
-                     "avg_launch_ms": avg_launch_s * 1000.0,
-                     "stage_ms": [ms1 / max(n1, 1), ms2 / max(n2, 1)]},
+        "config": {"workload": workload, "cells": pr.ncells, "blocks": pr.nblocks,
+                   "stages_per_step": 2},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not mhd and not args.no_ensemble:
+        out["batched"] = run_ensemble(args.steps, args.warmup)
+    if not mhd and not args.no_adjoint:
+        out["adjoint"] = run_adjoint()
     print(json.dumps(out), flush=True)
 
 

@@ -62,6 +62,22 @@ def load():
     lib.fx_mesh_view_get.argtypes = [vp, C.POINTER(MeshView)]
     lib.fx_mesh_set_timing.argtypes = [vp, C.c_int]
     lib.fx_mesh_kernel_time.argtypes = [vp, C.c_int, _dp, C.POINTER(C.c_int)]
+    lib.fx_ensemble_create.argtypes = [C.c_int, C.POINTER(MeshDesc), C.POINTER(vp)]
+    lib.fx_ensemble_destroy.argtypes = [vp]
+    lib.fx_ensemble_set_prim.argtypes = [vp, C.c_int, _dp]
+    lib.fx_ensemble_get_prim.argtypes = [vp, C.c_int, C.c_int, _dp]
+    lib.fx_ensemble_set_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.fx_ensemble_set_graph.argtypes = [vp, C.c_int]
+    lib.fx_ensemble_step_async.argtypes = [vp, C.c_double, C.c_double]
+    lib.fx_ensemble_sync.argtypes = [vp]
+    lib.fx_ensemble_last_dt.argtypes = [vp, _dp]
+    lib.fx_ensemble_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    lib.fx_ensemble_autotune.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int), _dp]
+    lib.fx_adj_create.argtypes = [C.POINTER(C.c_int), _dp, C.c_double, C.c_double, C.c_int,
+                                  C.POINTER(vp)]
+    lib.fx_adj_destroy.argtypes = [vp]
+    lib.fx_adj_forward.argtypes = [vp, vp, vp, C.c_int, C.c_double, vp]
+    lib.fx_adj_backward.argtypes = [vp, vp, vp, vp]
     lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
     lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
     lib.fx_emf_upwind_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
@@ -81,5 +97,9 @@ EXPORTS = [
     "fx_step_async", "fx_step", "fx_sync", "fx_last_dt", "fx_mesh_view_get",
     "fx_mesh_set_timing", "fx_mesh_kernel_time",
     "fx_riemann_hllc_batch", "fx_riemann_hlld_batch", "fx_emf_upwind_batch",
+    "fx_ensemble_create", "fx_ensemble_destroy", "fx_ensemble_set_prim", "fx_ensemble_get_prim",
+    "fx_ensemble_set_schedule", "fx_ensemble_set_graph", "fx_ensemble_step_async",
+    "fx_ensemble_sync", "fx_ensemble_last_dt", "fx_ensemble_info", "fx_ensemble_autotune",
+    "fx_adj_create", "fx_adj_destroy", "fx_adj_forward", "fx_adj_backward",
     "fx_last_error", "fx_version",
 ]

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/fornax_gpu.h"
@@ -462,6 +463,491 @@ int fx_emf_upwind_batch(int n, const double* emf_face, const double* emf_cell,
 {
     if (n < 0) return fail(FX_EINVAL, "negative count");
     FX_HIP(launch_emf_batch(n, emf_face, emf_cell, v_norm, emf, (hipStream_t)stream));
+    return FX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ragged ensembles of independent hydro meshes.
+struct fx_ensemble {
+    int n = 0;
+    std::vector<Geom> geoms;
+    std::vector<EnsInst> host_inst;
+    std::vector<EnsBlock> order[2];      // instance-major and XCD-interleaved tables
+    EnsInst* d_inst = nullptr;
+    EnsBlock* d_blocks[2] = {nullptr, nullptr};
+    int* d_queues = nullptr;
+    double* dev = nullptr;               // 4 doubles per instance
+    int64_t total_shell = 0;
+    int nblocks = 0;
+    hipStream_t stream = nullptr;        // owned: graph capture needs a non-null stream
+    // schedule: mode 0 = one launch set per instance, 1 = batched
+    int mode = 1;
+    int persistent_wgs = 0;              // 0 = one workgroup per block
+    int xcd = 1;                         // remap / per-XCD queues
+    int table = 0;                       // 0 instance-major, 1 interleaved
+    // hipGraph of one step (batched mode)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double graph_cfl = -1.0, graph_factor = -1.0;
+};
+
+namespace {
+
+void ens_drop_graph(fx_ensemble* e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+int ens_enqueue_step(fx_ensemble* e, double cfl, double factor)
+{
+    hipStream_t s = e->stream;
+    if (e->mode == 0) {
+        for (int i = 0; i < e->n; ++i) {
+            const EnsInst& I = e->host_inst[i];
+            FX_HIP(launch_calc_dt(I.g, I.W[0], cfl, I.dev + 2, I.dev + 0, I.dev + 1, factor, s));
+            for (int st = 1; st <= 2; ++st) {
+                double* Win = st == 1 ? I.W[0] : I.W[1];
+                FX_HIP(launch_boundary_fill(I.g, Win, nullptr, s));
+                HydroStageArgs a;
+                a.g = I.g;
+                a.Win = Win;
+                a.W0 = I.W[0];
+                a.Wout = st == 1 ? I.W[1] : I.W[0];
+                a.dt_ptr = I.dev;
+                a.dt = 0.0;
+                FX_HIP(launch_hydro_stage(a, st, s));
+            }
+        }
+        return FX_OK;
+    }
+    const EnsBlock* tbl = e->d_blocks[e->table];
+    FX_HIP(launch_ens_dt(e->d_inst, e->n, tbl, e->nblocks, cfl, factor, s));
+    for (int st = 1; st <= 2; ++st) {
+        FX_HIP(launch_ens_fill(e->d_inst, e->n, e->total_shell, st == 1 ? 0 : 1, s));
+        FX_HIP(launch_ens_stage(e->d_inst, tbl, e->nblocks, st, e->persistent_wgs, e->xcd, e->xcd,
+                                e->d_queues, s));
+    }
+    return FX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fx_ensemble_destroy(fx_ensemble* e)
+{
+    if (e == nullptr) return FX_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ens_drop_graph(e);
+    for (auto& I : e->host_inst) {
+        (void)hipFree(I.W[0]);
+        (void)hipFree(I.W[1]);
+    }
+    (void)hipFree(e->d_inst);
+    (void)hipFree(e->d_blocks[0]);
+    (void)hipFree(e->d_blocks[1]);
+    (void)hipFree(e->d_queues);
+    (void)hipFree(e->dev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return FX_OK;
+}
+
+int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
+{
+    if (n <= 0 || descs == nullptr || out == nullptr) return fail(FX_EINVAL, "bad ensemble arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        const fx_mesh_desc& d = descs[i];
+        if (d.mhd) return fail(FX_EINVAL, "ensembles are hydro only");
+        for (int a = 0; a < 3; ++a) {
+            if (d.n[a] <= 0 || d.n[a] % BS != 0)
+                return fail(FX_EINVAL, "grid extent must be a positive multiple of 16");
+            bool lo = d.bc[2 * a] == 3, hi = d.bc[2 * a + 1] == 3;
+            if (lo != hi) return fail(FX_EINVAL, "periodic boundaries must be paired");
+            if (lo && d.n[a] < 2 * BS) return fail(FX_EINVAL, "a periodic axis needs at least two blocks");
+        }
+        if (!(d.gamma > 1.0)) return fail(FX_EINVAL, "gamma must exceed 1");
+    }
+    fx_ensemble* e = new fx_ensemble();
+    e->n = n;
+    hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipMalloc(&e->dev, sizeof(double) * 4 * n);
+    if (err == hipSuccess) err = hipMemset(e->dev, 0, sizeof(double) * 4 * n);
+    int64_t shell = 0;
+    for (int i = 0; i < n && err == hipSuccess; ++i) {
+        const fx_mesh_desc& d = descs[i];
+        Geom g = make_geom(d.n[0], d.n[1], d.n[2], d.xlim, d.bc, d.gamma, 0, 0.0);
+        EnsInst I{};
+        I.g = g;
+        const size_t bytes = (size_t)g.vstride * 6 * sizeof(double);
+        err = hipMalloc(&I.W[0], bytes);
+        if (err == hipSuccess) err = hipMemset(I.W[0], 0, bytes);
+        if (err == hipSuccess) err = hipMalloc(&I.W[1], bytes);
+        if (err == hipSuccess) err = hipMemset(I.W[1], 0, bytes);
+        I.dev = e->dev + 4 * i;
+        I.shell_begin = shell;
+        I.shell_count = shell_cell_count(g);
+        shell += I.shell_count;
+        e->geoms.push_back(g);
+        e->host_inst.push_back(I);
+        for (int bx = 0; bx < g.nbx; ++bx)
+            for (int by = 0; by < g.nby; ++by)
+                for (int bz = 0; bz < g.nbz; ++bz) e->order[0].push_back({i, bx, by, bz});
+    }
+    e->total_shell = shell;
+    e->nblocks = (int)e->order[0].size();
+    // interleaved table: instances dealt round robin block by block, so that
+    // every part of the table mixes all instances
+    {
+        std::vector<size_t> next(n, 0), start(n, 0);
+        size_t acc = 0;
+        for (int i = 0; i < n; ++i) {
+            start[i] = acc;
+            acc += (size_t)e->geoms[i].nbx * e->geoms[i].nby * e->geoms[i].nbz;
+        }
+        bool left = true;
+        while (left) {
+            left = false;
+            for (int i = 0; i < n; ++i) {
+                size_t cnt = (size_t)e->geoms[i].nbx * e->geoms[i].nby * e->geoms[i].nbz;
+                if (next[i] < cnt) {
+                    e->order[1].push_back(e->order[0][start[i] + next[i]]);
+                    ++next[i];
+                    left = true;
+                }
+            }
+        }
+    }
+    if (err == hipSuccess) err = hipMalloc(&e->d_inst, sizeof(EnsInst) * n);
+    if (err == hipSuccess)
+        err = hipMemcpy(e->d_inst, e->host_inst.data(), sizeof(EnsInst) * n, hipMemcpyHostToDevice);
+    for (int t = 0; t < 2 && err == hipSuccess; ++t) {
+        err = hipMalloc(&e->d_blocks[t], sizeof(EnsBlock) * e->nblocks);
+        if (err == hipSuccess)
+            err = hipMemcpy(e->d_blocks[t], e->order[t].data(), sizeof(EnsBlock) * e->nblocks,
+                            hipMemcpyHostToDevice);
+    }
+    if (err == hipSuccess) err = hipMalloc(&e->d_queues, 64 * sizeof(int));
+    if (err == hipSuccess) err = hipMemset(e->d_queues, 0, 64 * sizeof(int));
+    if (err != hipSuccess) {
+        fx_ensemble_destroy(e);
+        return fail(err == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,
+                    std::string("fx_ensemble_create: ") + hipGetErrorString(err));
+    }
+    *out = e;
+    return FX_OK;
+}
+
+static int ens_check(fx_ensemble* e, int i)
+{
+    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    if (i < 0 || i >= e->n) return fail(FX_EINVAL, "instance index out of range");
+    return FX_OK;
+}
+
+int fx_ensemble_set_prim(fx_ensemble* e, int i, const double* W)
+{
+    if (int rc = ens_check(e, i)) return rc;
+    fx_mesh tmp;
+    tmp.g = e->geoms[i];
+    tmp.stream = e->stream;
+    for (int s = 0; s < 2; ++s)
+        if (int rc = copy_cells(&tmp, e->host_inst[i].W[s], W, nullptr, 6)) return rc;
+    double zero[4] = {0.0, 0.0, 0.0, 0.0};
+    FX_HIP(hipMemcpyAsync(e->host_inst[i].dev, zero, sizeof(zero), hipMemcpyHostToDevice, e->stream));
+    FX_HIP(hipStreamSynchronize(e->stream));
+    return FX_OK;
+}
+
+int fx_ensemble_get_prim(fx_ensemble* e, int i, int slot, double* W)
+{
+    if (int rc = ens_check(e, i)) return rc;
+    if (slot != 0 && slot != 1) return fail(FX_EINVAL, "bad slot");
+    fx_mesh tmp;
+    tmp.g = e->geoms[i];
+    tmp.stream = e->stream;
+    return copy_cells(&tmp, e->host_inst[i].W[slot], nullptr, W, 6);
+}
+
+int fx_ensemble_set_schedule(fx_ensemble* e, int mode, int persistent_wgs, int xcd, int table)
+{
+    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    if (mode < 0 || mode > 1 || persistent_wgs < 0 || table < 0 || table > 1)
+        return fail(FX_EINVAL, "bad schedule");
+    FX_HIP(hipStreamSynchronize(e->stream));
+    e->mode = mode;
+    e->persistent_wgs = persistent_wgs;
+    e->xcd = xcd ? 1 : 0;
+    e->table = table;
+    ens_drop_graph(e);
+    return FX_OK;
+}
+
+int fx_ensemble_set_graph(fx_ensemble* e, int on)
+{
+    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    FX_HIP(hipStreamSynchronize(e->stream));
+    e->use_graph = on != 0;
+    ens_drop_graph(e);
+    return FX_OK;
+}
+
+int fx_ensemble_step_async(fx_ensemble* e, double cfl, double dt_factor)
+{
+    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    if (!e->use_graph) return ens_enqueue_step(e, cfl, dt_factor);
+    if (e->exec == nullptr || e->graph_cfl != cfl || e->graph_factor != dt_factor) {
+        ens_drop_graph(e);
+        FX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = ens_enqueue_step(e, cfl, dt_factor);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        FX_HIP(ce);
+        e->graph = g;
+        FX_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+        e->graph_cfl = cfl;
+        e->graph_factor = dt_factor;
+    }
+    FX_HIP(hipGraphLaunch(e->exec, e->stream));
+    return FX_OK;
+}
+
+int fx_ensemble_sync(fx_ensemble* e)
+{
+    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    FX_HIP(hipStreamSynchronize(e->stream));
+    return FX_OK;
+}
+
+int fx_ensemble_last_dt(fx_ensemble* e, double* dt)
+{
+    if (e == nullptr || dt == nullptr) return fail(FX_EINVAL, "null argument");
+    std::vector<double> h(4 * (size_t)e->n);
+    FX_HIP(hipMemcpyAsync(h.data(), e->dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost, e->stream));
+    FX_HIP(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < e->n; ++i) dt[i] = h[4 * (size_t)i];
+    return FX_OK;
+}
+
+int fx_ensemble_info(fx_ensemble* e, int* nblocks, long long* ncells)
+{
+    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    if (nblocks) *nblocks = e->nblocks;
+    if (ncells) *ncells = (long long)e->nblocks * BS * BS * BS;
+    return FX_OK;
+}
+
+// Time `reps` steps of every candidate schedule on the current state (the
+// state advances; callers tune on a copy or re-set it) and keep the fastest.
+// out[0..3] = mode, persistent_wgs, xcd, table; *best_ms = ms per step.
+int fx_ensemble_autotune(fx_ensemble* e, double cfl, double dt_factor, int reps, int* out,
+                         double* best_ms)
+{
+    if (e == nullptr || reps <= 0) return fail(FX_EINVAL, "bad autotune arguments");
+    int dev_id = 0;
+    FX_HIP(hipGetDevice(&dev_id));
+    int ncu = 0;
+    FX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_id));
+    const int cand[][4] = {
+        {1, 0, 1, 0}, {1, 0, 0, 0}, {1, 0, 1, 1},
+        {1, ncu, 1, 0}, {1, 2 * ncu, 1, 0}, {1, 2 * ncu, 0, 1},
+    };
+    const bool graph = e->use_graph;
+    double best = 1e300;
+    int bi = 0;
+    struct Events {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    FX_HIP(hipEventCreate(&ev.a));
+    FX_HIP(hipEventCreate(&ev.b));
+    hipEvent_t a = ev.a, b = ev.b;
+    for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); ++c) {
+        if (int rc = fx_ensemble_set_schedule(e, cand[c][0], cand[c][1], cand[c][2], cand[c][3])) return rc;
+        e->use_graph = graph;
+        if (int rc = fx_ensemble_step_async(e, cfl, dt_factor)) return rc;   // warm-up / capture
+        FX_HIP(hipEventRecord(a, e->stream));
+        for (int r = 0; r < reps; ++r)
+            if (int rc = fx_ensemble_step_async(e, cfl, dt_factor)) return rc;
+        FX_HIP(hipEventRecord(b, e->stream));
+        FX_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        FX_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms < best) {
+            best = ms;
+            bi = c;
+        }
+    }
+    if (int rc = fx_ensemble_set_schedule(e, cand[bi][0], cand[bi][1], cand[bi][2], cand[bi][3])) return rc;
+    e->use_graph = graph;
+    if (out) for (int q = 0; q < 4; ++q) out[q] = cand[bi][q];
+    if (best_ms) *best_ms = best / reps;
+    return FX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Differentiable scheme: forward with checkpoints, reverse sweep with recompute.
+struct fx_adj {
+    AdjGeom g;
+    int every = 8;                        // checkpoint interval (steps)
+    size_t state_bytes = 0;               // 6 * N doubles
+    double* S = nullptr;                  // running state; W_T after a forward
+    double* W1 = nullptr;                 // stage-1 output of the plain forward
+    double* F = nullptr;                  // 18 N
+    double* G = nullptr;                  // 36 N
+    double* Ub[2] = {nullptr, nullptr};   // 6 N each
+    double* seg = nullptr;                // 2 * every states of one recomputed segment
+    double* ckpt = nullptr;               // checkpoints of the last forward
+    int ckpt_cap = 0;
+    int nsteps = -1;                      // of the last forward (-1: none)
+    double dt = 0.0;
+};
+
+namespace {
+
+inline double* adj_state(double* base, const fx_adj* a, int i)
+{
+    return base + (size_t)i * 6 * (size_t)a->g.N;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fx_adj_destroy(fx_adj* a)
+{
+    if (a == nullptr) return FX_OK;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(a->S);
+    (void)hipFree(a->W1);
+    (void)hipFree(a->F);
+    (void)hipFree(a->G);
+    (void)hipFree(a->Ub[0]);
+    (void)hipFree(a->Ub[1]);
+    (void)hipFree(a->seg);
+    (void)hipFree(a->ckpt);
+    delete a;
+    return FX_OK;
+}
+
+int fx_adj_create(const int* n, const double* xlim, double gamma, double va_eps2,
+                  int checkpoint_every, fx_adj** out)
+{
+    if (n == nullptr || xlim == nullptr || out == nullptr) return fail(FX_EINVAL, "null argument");
+    *out = nullptr;
+    for (int d = 0; d < 3; ++d)
+        if (n[d] < 4) return fail(FX_EINVAL, "the periodic box needs at least 4 cells per axis");
+    if (!(gamma > 1.0)) return fail(FX_EINVAL, "gamma must exceed 1");
+    if (!(va_eps2 > 0.0)) return fail(FX_EINVAL, "van Albada eps^2 must be positive");
+    if (checkpoint_every < 1) return fail(FX_EINVAL, "checkpoint interval must be >= 1");
+    fx_adj* a = new fx_adj();
+    AdjGeom& g = a->g;
+    for (int d = 0; d < 3; ++d) {
+        g.n[d] = n[d];
+        g.dx[d] = (xlim[2 * d + 1] - xlim[2 * d]) / (double)n[d];
+    }
+    g.N = (int64_t)n[0] * n[1] * n[2];
+    g.vol = g.dx[0] * g.dx[1] * g.dx[2];
+    g.area[0] = g.dx[1] * g.dx[2];
+    g.area[1] = g.dx[0] * g.dx[2];
+    g.area[2] = g.dx[0] * g.dx[1];
+    g.gamma = gamma;
+    g.eps2 = va_eps2;
+    a->every = checkpoint_every;
+    a->state_bytes = sizeof(double) * 6 * (size_t)g.N;
+    hipError_t e = hipMalloc(&a->S, a->state_bytes);
+    if (e == hipSuccess) e = hipMalloc(&a->W1, a->state_bytes);
+    if (e == hipSuccess) e = hipMalloc(&a->F, 3 * a->state_bytes);
+    if (e == hipSuccess) e = hipMalloc(&a->G, 6 * a->state_bytes);
+    if (e == hipSuccess) e = hipMalloc(&a->Ub[0], a->state_bytes);
+    if (e == hipSuccess) e = hipMalloc(&a->Ub[1], a->state_bytes);
+    if (e == hipSuccess) e = hipMalloc(&a->seg, 2 * (size_t)checkpoint_every * a->state_bytes);
+    if (e != hipSuccess) {
+        fx_adj_destroy(a);
+        return fail(e == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,
+                    std::string("fx_adj_create: ") + hipGetErrorString(e));
+    }
+    *out = a;
+    return FX_OK;
+}
+
+int fx_adj_forward(fx_adj* a, const double* W0, double* WT, int nsteps, double dt, void* stream)
+{
+    if (a == nullptr || W0 == nullptr || WT == nullptr) return fail(FX_EINVAL, "null argument");
+    if (nsteps < 0) return fail(FX_EINVAL, "negative step count");
+    hipStream_t s = (hipStream_t)stream;
+    const int nck = nsteps / a->every + 1;
+    if (nck > a->ckpt_cap) {
+        FX_HIP(hipStreamSynchronize(s));
+        (void)hipFree(a->ckpt);
+        a->ckpt = nullptr;
+        a->ckpt_cap = 0;
+        FX_HIP(hipMalloc(&a->ckpt, (size_t)nck * a->state_bytes));
+        a->ckpt_cap = nck;
+    }
+    a->nsteps = -1;
+    FX_HIP(hipMemcpyAsync(a->S, W0, a->state_bytes, hipMemcpyDeviceToDevice, s));
+    for (int k = 0; k < nsteps; ++k) {
+        if (k % a->every == 0)
+            FX_HIP(hipMemcpyAsync(adj_state(a->ckpt, a, k / a->every), a->S, a->state_bytes,
+                                  hipMemcpyDeviceToDevice, s));
+        FX_HIP(launch_adj_stage(a->g, 1, dt, a->S, a->S, a->F, a->W1, s));
+        FX_HIP(launch_adj_stage(a->g, 2, dt, a->W1, a->S, a->F, a->S, s));
+    }
+    FX_HIP(hipMemcpyAsync(WT, a->S, a->state_bytes, hipMemcpyDeviceToDevice, s));
+    a->nsteps = nsteps;
+    a->dt = dt;
+    return FX_OK;
+}
+
+int fx_adj_backward(fx_adj* a, const double* gWT, double* gW0, void* stream)
+{
+    if (a == nullptr || gWT == nullptr || gW0 == nullptr) return fail(FX_EINVAL, "null argument");
+    if (a->nsteps < 0) return fail(FX_EINVAL, "fx_adj_backward needs a preceding fx_adj_forward");
+    hipStream_t s = (hipStream_t)stream;
+    const AdjGeom& g = a->g;
+    const int n = a->nsteps, K = a->every;
+    double* Ub2 = a->Ub[0];
+    double* Ub1 = a->Ub[1];
+    FX_HIP(launch_adj_w2u(g, a->S, gWT, Ub2, s));
+    for (int sg = (n - 1) / K; n > 0 && sg >= 0; --sg) {
+        const int k0 = sg * K, k1 = std::min(n, k0 + K), m = k1 - k0;
+        // recompute the segment's stage inputs: seg[2k] = W^n, seg[2k+1] = W^(1)
+        FX_HIP(hipMemcpyAsync(adj_state(a->seg, a, 0), adj_state(a->ckpt, a, sg), a->state_bytes,
+                              hipMemcpyDeviceToDevice, s));
+        for (int k = 0; k < m; ++k) {
+            FX_HIP(launch_adj_stage(g, 1, a->dt, adj_state(a->seg, a, 2 * k),
+                                    adj_state(a->seg, a, 2 * k), a->F,
+                                    adj_state(a->seg, a, 2 * k + 1), s));
+            if (k + 1 < m)
+                FX_HIP(launch_adj_stage(g, 2, a->dt, adj_state(a->seg, a, 2 * k + 1),
+                                        adj_state(a->seg, a, 2 * k), a->F,
+                                        adj_state(a->seg, a, 2 * k + 2), s));
+        }
+        for (int k = m - 1; k >= 0; --k)
+            FX_HIP(launch_adj_step_reverse(g, a->dt, adj_state(a->seg, a, 2 * k),
+                                           adj_state(a->seg, a, 2 * k + 1), Ub2, Ub1, a->G, s));
+    }
+    // initial state: W0 is checkpoint 0 (or S itself when nsteps == 0)
+    const double* W0 = n > 0 ? adj_state(a->ckpt, a, 0) : a->S;
+    FX_HIP(launch_adj_u2w(g, W0, Ub2, gW0, s));
     return FX_OK;
 }
 

@@ -108,33 +108,43 @@ __device__ __forceinline__ int block_of_workgroup(int wg, int nwg)
     return wg;
 }
 
-template <int STAGE>
-__global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
-{
-    // planes i and i+1 (rho v1 v2 v3 e ye; pressure is formed from rho*e on
-    // use), v1 of plane i-1, and the face fluxes that cross a wave boundary
-    __shared__ double pl[2][NV][PC];
-    __shared__ double v1p[PC];
-    __shared__ double exup[6][4][BS];   // Fy(4(w+1), k) for the last row of wave w
-    __shared__ double ez16[6][BS];      // Fz(j, 16): the block's high-side z faces
-    __shared__ double hy[NQ][288];      // MC half-slopes along y, cells j=-1..16, k=0..15
-    __shared__ double hz[NQ][288];      // MC half-slopes along z, cells j=0..15, k=-1..16
+// LDS of one block's stage: planes i and i+1 (rho v1 v2 v3 e ye; pressure is
+// formed from rho*e on use), v1 of plane i-1, the face fluxes that cross a
+// wave boundary, and the per-cell MC half-slopes of plane i along y and z.
+struct HydroLds {
+    double pl[2][NV][PC];
+    double v1p[PC];
+    double exup[6][4][BS];   // Fy(4(w+1), k) for the last row of wave w
+    double ez16[6][BS];      // Fz(j, 16): the block's high-side z faces
+    double hy[NQ][288];      // MC half-slopes along y, cells j=-1..16, k=0..15
+    double hz[NQ][288];      // MC half-slopes along z, cells j=0..15, k=-1..16
+};
 
-    const Geom& g = a.g;
+// One RK2 stage of block (bx, by, bz) of the mesh described by g.  Called by
+// the single-mesh kernel and by the batched ensemble kernels.
+template <int STAGE>
+__device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
+                                            const double* __restrict__ Win,
+                                            const double* __restrict__ W0,
+                                            double* __restrict__ Wout, double dt, int bx, int by,
+                                            int bz)
+{
+    auto& pl = L.pl;
+    auto& v1p = L.v1p;
+    auto& exup = L.exup;
+    auto& ez16 = L.ez16;
+    auto& hy = L.hy;
+    auto& hz = L.hz;
+
     const int t = threadIdx.x;
     const int j = t >> 4;
     const int k = t & 15;
     const int lane = t & 63;
     const int wave = t >> 6;
-    const int b = block_of_workgroup(blockIdx.x, gridDim.x);
-    const int bz = b % g.nbz;
-    const int by = (b / g.nbz) % g.nby;
-    const int bx = b / (g.nbz * g.nby);
     const int x0 = bx * BS, y0 = by * BS, z0 = bz * BS;
     const int64_t vs = g.vstride;
     const double gm1 = g.gamma - 1.0;
     const double gam = g.gamma;
-    const double dt = a.dt_ptr ? *a.dt_ptr : a.dt;
 
     // own cell and (for t < 144) one halo cell of the 20x20 plane
     const int oidx = (j + NG) * PW + (k + NG);
@@ -165,19 +175,19 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
     {
         double cmm[NQ], cm[NQ], c0[NQ], c1[NQ];
         PlaneRegs r;
-        load_plane(a.Win, vs, own_at(-1), halo_at(-1), has_halo, r);
+        load_plane(Win, vs, own_at(-1), halo_at(-1), has_halo, r);
         store_plane(pl[1], r, oidx, hidx, has_halo);        // plane -1 (temporary)
         v1p[oidx] = r.own[1];
         if (has_halo) v1p[hidx] = r.halo[1];
 #pragma unroll
         for (int q = 0; q < NV; ++q) cm[q] = r.own[q];
-        load_plane(a.Win, vs, own_at(0), halo_at(0), has_halo, r);
+        load_plane(Win, vs, own_at(0), halo_at(0), has_halo, r);
         store_plane(pl[0], r, oidx, hidx, has_halo);        // plane 0
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             c0[q] = r.own[q];
-            cmm[q] = a.Win[q * vs + own_at(-2)];
-            c1[q] = a.Win[q * vs + own_at(1)];
+            cmm[q] = Win[q * vs + own_at(-2)];
+            c1[q] = Win[q * vs + own_at(1)];
         }
         cmm[6] = gm1 * cmm[0] * cmm[4];
         cm[6] = gm1 * cm[0] * cm[4];
@@ -208,9 +218,9 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
         double vf[3];
         hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fx, vf);
         __syncthreads();
-        load_plane(a.Win, vs, own_at(1), halo_at(1), has_halo, r);
+        load_plane(Win, vs, own_at(1), halo_at(1), has_halo, r);
         store_plane(pl[1], r, oidx, hidx, has_halo);        // plane 1
-        load_plane(a.Win, vs, own_at(2), halo_at(2), has_halo, pf);
+        load_plane(Win, vs, own_at(2), halo_at(2), has_halo, pf);
         __syncthreads();
     }
 
@@ -434,7 +444,7 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
             } else {
                 double us[6], w0[6];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) w0[q] = a.W0[q * vs + o];
+                for (int q = 0; q < 6; ++q) w0[q] = W0[q * vs + o];
                 prim2cons6(w0, u);
                 prim2cons6(wi, us);
 #pragma unroll
@@ -445,14 +455,84 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
             }
             cons2prim6(u1, w);
 #pragma unroll
-            for (int v = 0; v < 6; ++v) __builtin_nontemporal_store(w[v], &a.Wout[v * vs + o]);
+            for (int v = 0; v < 6; ++v) __builtin_nontemporal_store(w[v], &Wout[v * vs + o]);
         }
 
 #pragma unroll
         for (int q = 0; q < 6; ++q) Fx[q] = Fxn[q];
-        if (i + 3 <= BS + 1) load_plane(a.Win, vs, own_at(i + 3), halo_at(i + 3), has_halo, pf);
+        if (i + 3 <= BS + 1) load_plane(Win, vs, own_at(i + 3), halo_at(i + 3), has_halo, pf);
         __syncthreads();
     }
+}
+
+template <int STAGE>
+__global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
+{
+    __shared__ HydroLds L;
+    const Geom& g = a.g;
+    const int b = block_of_workgroup(blockIdx.x, gridDim.x);
+    const int bz = b % g.nbz;
+    const int by = (b / g.nbz) % g.nby;
+    const int bx = b / (g.nbz * g.nby);
+    const double dt = a.dt_ptr ? *a.dt_ptr : a.dt;
+    hydro_block<STAGE>(L, g, a.Win, a.W0, a.Wout, dt, bx, by, bz);
+}
+
+// One table entry of a batched stage: block (bx, by, bz) of instance `inst`.
+template <int STAGE>
+__device__ __forceinline__ void ens_entry(HydroLds& L, const EnsInst* __restrict__ inst,
+                                          const EnsBlock* __restrict__ blocks, int b)
+{
+    const EnsBlock e = blocks[b];
+    const EnsInst& I = inst[e.inst];
+    const double dt = I.dev[0];
+    hydro_block<STAGE>(L, I.g, I.W[STAGE == 1 ? 0 : 1], I.W[0], I.W[STAGE == 1 ? 1 : 0], dt,
+                       e.bx, e.by, e.bz);
+}
+
+// Batched stage over a table of (instance, block) entries, one workgroup per
+// entry; XCD_REMAP gives every XCD a contiguous run of the table.
+template <int STAGE, int XCD_REMAP>
+__global__ void __launch_bounds__(NT) ens_stage_kernel(const EnsInst* __restrict__ inst,
+                                                      const EnsBlock* __restrict__ blocks)
+{
+    __shared__ HydroLds L;
+    const int b = XCD_REMAP ? block_of_workgroup(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    ens_entry<STAGE>(L, inst, blocks, b);
+}
+
+// Persistent batched stage: a fixed number of workgroups pull table entries
+// from atomic counters until the table is exhausted.  With XCD_QUEUES the
+// table is cut into 8 contiguous chunks, one counter each, and workgroup w
+// (dispatched to XCD w mod 8) drains chunk w mod 8 first, then the others.
+// Every workgroup leaves once all counters have passed their chunk ends.
+template <int STAGE, int XCD_QUEUES>
+__global__ void __launch_bounds__(NT) ens_stage_persistent_kernel(const EnsInst* __restrict__ inst,
+                                                                 const EnsBlock* __restrict__ blocks,
+                                                                 int nblocks, int* queues)
+{
+    __shared__ HydroLds L;
+    __shared__ int s_b;
+    const int nq = XCD_QUEUES ? 8 : 1;
+    const int per = (nblocks + nq - 1) / nq;
+    for (int r = 0; r < nq; ++r) {
+        const int q = XCD_QUEUES ? (int)((blockIdx.x + r) & 7) : 0;
+        const int lo = q * per;
+        const int hi = min(nblocks, lo + per);
+        for (;;) {
+            if (threadIdx.x == 0) s_b = lo + atomicAdd(&queues[q], 1);
+            __syncthreads();
+            const int b = __builtin_amdgcn_readfirstlane(s_b);
+            __syncthreads();
+            if (b >= hi) break;
+            ens_entry<STAGE>(L, inst, blocks, b);
+        }
+    }
+}
+
+__global__ void queue_reset_kernel(int* q)
+{
+    if (threadIdx.x < 8) q[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -464,6 +544,34 @@ hipError_t launch_hydro_stage(const HydroStageArgs& a, int stage, hipStream_t st
         hipLaunchKernelGGL(hydro_stage_kernel<1>, dim3(nblocks), dim3(NT), 0, stream, a);
     } else {
         hipLaunchKernelGGL(hydro_stage_kernel<2>, dim3(nblocks), dim3(NT), 0, stream, a);
+    }
+    return hipGetLastError();
+}
+
+
+hipError_t launch_ens_stage(const EnsInst* inst, const EnsBlock* blocks, int nblocks, int stage,
+                            int persistent_wgs, int xcd_remap, int xcd_queues, int* queues,
+                            hipStream_t s)
+{
+    if (nblocks <= 0) return hipSuccess;
+    if (persistent_wgs <= 0) {
+        if (stage == 1) {
+            if (xcd_remap) hipLaunchKernelGGL((ens_stage_kernel<1, 1>), dim3(nblocks), dim3(NT), 0, s, inst, blocks);
+            else hipLaunchKernelGGL((ens_stage_kernel<1, 0>), dim3(nblocks), dim3(NT), 0, s, inst, blocks);
+        } else {
+            if (xcd_remap) hipLaunchKernelGGL((ens_stage_kernel<2, 1>), dim3(nblocks), dim3(NT), 0, s, inst, blocks);
+            else hipLaunchKernelGGL((ens_stage_kernel<2, 0>), dim3(nblocks), dim3(NT), 0, s, inst, blocks);
+        }
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, queues);
+    const dim3 grid(persistent_wgs);
+    if (stage == 1) {
+        if (xcd_queues) hipLaunchKernelGGL((ens_stage_persistent_kernel<1, 1>), grid, dim3(NT), 0, s, inst, blocks, nblocks, queues);
+        else hipLaunchKernelGGL((ens_stage_persistent_kernel<1, 0>), grid, dim3(NT), 0, s, inst, blocks, nblocks, queues);
+    } else {
+        if (xcd_queues) hipLaunchKernelGGL((ens_stage_persistent_kernel<2, 1>), grid, dim3(NT), 0, s, inst, blocks, nblocks, queues);
+        else hipLaunchKernelGGL((ens_stage_persistent_kernel<2, 0>), grid, dim3(NT), 0, s, inst, blocks, nblocks, queues);
     }
     return hipGetLastError();
 }

@@ -31,6 +31,33 @@ struct MhdStageArgs {
     double dt;
 };
 
+// ---- ragged ensembles of independent hydro meshes --------------------------
+struct EnsInst {
+    Geom g;
+    double* W[2];           // stage slots of this instance (padded layout)
+    double* dev;            // [0] dt, [1] dt_prev, [2] CFL reduction (bits of a positive double)
+    int64_t shell_begin;    // first index of this instance in the batched ghost-shell enumeration
+    int64_t shell_count;
+};
+
+struct EnsBlock {
+    int inst, bx, by, bz;
+};
+
+// number of ghost-shell cells of one mesh (the domain-boundary fill)
+int64_t shell_cell_count(const Geom& g);
+
+hipError_t launch_ens_fill(const EnsInst* inst, int ninst, int64_t total_shell, int slot,
+                           hipStream_t s);
+hipError_t launch_ens_dt(const EnsInst* inst, int ninst, const EnsBlock* blocks, int nblocks,
+                         double cfl, double factor, hipStream_t s);
+// persistent_wgs == 0: one workgroup per table entry (XCD remap if xcd_remap);
+// otherwise that many persistent workgroups pull table entries from `queues`
+// (8 counters, one per XCD chunk when xcd_queues, else only the first).
+hipError_t launch_ens_stage(const EnsInst* inst, const EnsBlock* blocks, int nblocks, int stage,
+                            int persistent_wgs, int xcd_remap, int xcd_queues, int* queues,
+                            hipStream_t s);
+
 // fill the two-cell ghost shell of the domain (periodic / outflow / reflect;
 // MHD: cell B from the clamped outflow face fields)
 hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s);
@@ -59,5 +86,27 @@ hipError_t launch_hlld_batch(int n, const double* WL, const double* WR, const do
                              double* bv2, int* status, hipStream_t s);
 hipError_t launch_emf_batch(int n, const double* ef, const double* ec, const double* vn,
                             double* emf, hipStream_t s);
+
+// ---- differentiable scheme (van Albada + HLLE) and its adjoint --------------
+// compact periodic layout [var][X][Y][Z], N = X*Y*Z
+struct AdjGeom {
+    int n[3];
+    int64_t N;
+    double dx[3], area[3], vol;
+    double gamma;
+    double eps2;            // van Albada regularisation
+};
+
+// one forward RK2 stage: F scratch 18*N; stage 2 may write Wout == W0
+hipError_t launch_adj_stage(const AdjGeom& g, int stage, double dt, const double* Win,
+                            const double* W0, double* F, double* Wout, hipStream_t s);
+hipError_t launch_adj_w2u(const AdjGeom& g, const double* W, const double* Wb, double* Ub,
+                          hipStream_t s);
+hipError_t launch_adj_u2w(const AdjGeom& g, const double* W, const double* Ub, double* Wb,
+                          hipStream_t s);
+// reverse of one RK2 step (W0, W1 its stage inputs); U2b in: cotangent of the
+// step's output in conserved form, out: that of W0; U1b 6*N and G 36*N scratch
+hipError_t launch_adj_step_reverse(const AdjGeom& g, double dt, const double* W0, const double* W1,
+                                   double* U2b, double* U1b, double* G, hipStream_t s);
 
 }  // namespace fx

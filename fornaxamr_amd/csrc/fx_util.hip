@@ -9,6 +9,8 @@ namespace fx {
 
 namespace {
 
+inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
 // Ghost mapping of one axis: periodic wrap (the wrapped neighbour of
 // src/prj_mesh.c:1597-1645) or the mirror copy of the physical boundary
 // (src/prj_boundary.c:413-487: src = -a-1 / 2N-1-a), reflect negating the
@@ -34,12 +36,13 @@ __device__ __forceinline__ double bf_clamped(const Geom& g, const double* bf, in
     return bf[dir * g.fvstride + cell_off(g, x, y, z)];
 }
 
-__global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const double* __restrict__ bf,
-                                     int64_t nx_part, int64_t ny_part, int64_t total)
+// One cell of the ghost shell, enumerated as id in [0, shell_cell_count(g)).
+__device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restrict__ W,
+                                                const double* __restrict__ bf, int64_t id)
 {
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= total) return;
-    const int nxp = g.nx + 2 * NG, nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
+    const int nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
+    const int64_t nx_part = 4 * (int64_t)nyp * nzp;
+    const int64_t ny_part = 4 * (int64_t)g.nx * nzp;
     int x, y, z;
     if (id < nx_part) {
         int64_t r = id;
@@ -86,9 +89,59 @@ __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const doubl
     }
 }
 
+__global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const double* __restrict__ bf,
+                                     int64_t total)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    fill_shell_cell(g, W, bf, id);
+}
+
+// batched fill over the shells of all instances of an ensemble
+__global__ void ens_fill_kernel(const EnsInst* __restrict__ inst, int ninst, int64_t total, int slot)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {                    // last instance with shell_begin <= id
+        int mid = (lo + hi + 1) >> 1;
+        if (inst[mid].shell_begin <= id) lo = mid; else hi = mid - 1;
+    }
+    const EnsInst& I = inst[lo];
+    fill_shell_cell(I.g, I.W[slot], nullptr, id - I.shell_begin);
+}
+
 __global__ void dt_init_kernel(unsigned long long* s)
 {
     s[0] = (unsigned long long)__double_as_longlong(1.0e99);
+}
+
+// CFL denominator of one active cell (src/prj_timeint.c:2163-2182)
+__device__ __forceinline__ double cell_dt(const Geom& g, const double* __restrict__ W, int64_t o,
+                                          double cfl)
+{
+    const double gm1 = g.gamma - 1.0;
+    double rho = W[o];
+    double p = gm1 * rho * W[4 * g.vstride + o];
+    double cs = sqrt(g.gamma * p / rho);
+    double denom = (fabs(W[g.vstride + o]) + cs) / g.dx[0] +
+                   (fabs(W[2 * g.vstride + o]) + cs) / g.dx[1] +
+                   (fabs(W[3 * g.vstride + o]) + cs) / g.dx[2];
+    return cfl / denom;
+}
+
+__device__ __forceinline__ double block_min(double v, double* red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double o = red[threadIdx.x + w];
+            if (o < red[threadIdx.x]) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    return red[0];
 }
 
 // src/prj_timeint.c:2144-2256 (sound speed only, also for MHD)
@@ -98,41 +151,25 @@ __global__ void __launch_bounds__(256) dt_reduce_kernel(Geom g, const double* __
     __shared__ double red[256];
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     double best = 1.0e99;
-    const double gm1 = g.gamma - 1.0;
     for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
          id += (int64_t)gridDim.x * blockDim.x) {
         int z = (int)(id % g.nz);
         int64_t r = id / g.nz;
         int y = (int)(r % g.ny);
         int x = (int)(r / g.ny);
-        int64_t o = cell_off(g, x, y, z);
-        double rho = W[o];
-        double p = gm1 * rho * W[4 * g.vstride + o];
-        double cs = sqrt(g.gamma * p / rho);
-        double denom = (fabs(W[g.vstride + o]) + cs) / g.dx[0] +
-                       (fabs(W[2 * g.vstride + o]) + cs) / g.dx[1] +
-                       (fabs(W[3 * g.vstride + o]) + cs) / g.dx[2];
-        double dt_cell = cfl / denom;
+        double dt_cell = cell_dt(g, W, cell_off(g, x, y, z), cfl);
         if (dt_cell < best) best = dt_cell;
     }
-    red[threadIdx.x] = best;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            double o = red[threadIdx.x + w];
-            if (o < red[threadIdx.x]) red[threadIdx.x] = o;
-        }
-        __syncthreads();
-    }
+    best = block_min(best, red);
     if (threadIdx.x == 0) {
         // dt > 0: IEEE order of positive doubles is the order of their bits
-        atomicMin(s, (unsigned long long)__double_as_longlong(red[0]));
+        atomicMin(s, (unsigned long long)__double_as_longlong(best));
     }
 }
 
 // dt = min(dt_new, factor * dt_prev) after the first step (src/main.c:517-522)
-__global__ void dt_finalize_kernel(const unsigned long long* s, double* dt_out, double* dt_prev,
-                                   double factor)
+__device__ __forceinline__ void dt_finalize(const unsigned long long* s, double* dt_out,
+                                            double* dt_prev, double factor)
 {
     double dt_new = __longlong_as_double((long long)s[0]);
     double dt = dt_new;
@@ -145,6 +182,49 @@ __global__ void dt_finalize_kernel(const unsigned long long* s, double* dt_out, 
         dt_prev[0] = dt;
     }
     dt_out[0] = dt;
+}
+
+__global__ void dt_finalize_kernel(const unsigned long long* s, double* dt_out, double* dt_prev,
+                                   double factor)
+{
+    dt_finalize(s, dt_out, dt_prev, factor);
+}
+
+__global__ void ens_dt_init_kernel(const EnsInst* __restrict__ inst, int ninst)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ninst)
+        inst[i].dev[2] = __longlong_as_double(__double_as_longlong(1.0e99));
+}
+
+// one workgroup per block of the table: 256 threads x 16 cells along x
+__global__ void __launch_bounds__(256) ens_dt_reduce_kernel(const EnsInst* __restrict__ inst,
+                                                            const EnsBlock* __restrict__ blocks,
+                                                            double cfl)
+{
+    __shared__ double red[256];
+    const EnsBlock e = blocks[blockIdx.x];
+    const EnsInst& I = inst[e.inst];
+    const int j = threadIdx.x >> 4, k = threadIdx.x & 15;
+    double best = 1.0e99;
+    for (int i = 0; i < BS; ++i) {
+        int64_t o = cell_off(I.g, e.bx * BS + i, e.by * BS + j, e.bz * BS + k);
+        double d = cell_dt(I.g, I.W[0], o, cfl);
+        if (d < best) best = d;
+    }
+    best = block_min(best, red);
+    if (threadIdx.x == 0)
+        atomicMin(reinterpret_cast<unsigned long long*>(I.dev + 2),
+                  (unsigned long long)__double_as_longlong(best));
+}
+
+__global__ void ens_dt_finalize_kernel(const EnsInst* __restrict__ inst, int ninst, double factor)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ninst) {
+        double* d = inst[i].dev;
+        dt_finalize(reinterpret_cast<const unsigned long long*>(d + 2), d, d + 1, factor);
+    }
 }
 
 __global__ void hllc_batch_kernel(int n, const double* WL, const double* WR, const double* pL,
@@ -210,19 +290,19 @@ __global__ void emf_batch_kernel(int n, const double* ef, const double* ec, cons
     emf[f] = emf_upwind(a, c, v);
 }
 
-inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-
 }  // namespace
+
+int64_t shell_cell_count(const Geom& g)
+{
+    const int64_t nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
+    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 4 * (int64_t)g.nx * g.ny;
+}
 
 hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s)
 {
-    const int64_t nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
-    const int64_t nx_part = 4 * nyp * nzp;
-    const int64_t ny_part = 4 * (int64_t)g.nx * nzp;
-    const int64_t nz_part = 4 * (int64_t)g.nx * g.ny;
-    const int64_t total = nx_part + ny_part + nz_part;
+    const int64_t total = shell_cell_count(g);
     hipLaunchKernelGGL(boundary_fill_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, g, W,
-                       bf, nx_part, ny_part, total);
+                       bf, total);
     return hipGetLastError();
 }
 
@@ -236,6 +316,24 @@ hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* sc
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(dt_reduce_kernel, dim3(blocks), dim3(256), 0, s, g, W, cfl, acc);
     hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, s, acc, dt_out, dt_prev, factor);
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_fill(const EnsInst* inst, int ninst, int64_t total_shell, int slot,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(ens_fill_kernel, dim3(grid_for(total_shell, 256)), dim3(256), 0, s, inst,
+                       ninst, total_shell, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_dt(const EnsInst* inst, int ninst, const EnsBlock* blocks, int nblocks,
+                         double cfl, double factor, hipStream_t s)
+{
+    hipLaunchKernelGGL(ens_dt_init_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst, ninst);
+    hipLaunchKernelGGL(ens_dt_reduce_kernel, dim3(nblocks), dim3(256), 0, s, inst, blocks, cfl);
+    hipLaunchKernelGGL(ens_dt_finalize_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst,
+                       ninst, factor);
     return hipGetLastError();
 }
 

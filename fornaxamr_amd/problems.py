@@ -176,3 +176,60 @@ def random_smooth(n=(32, 32, 32), seed=0, bc=(PERIODIC,) * 6, gamma=1.4, mhd=Fal
         W[7] = 0.5 * (bf[1][:, :-1] + bf[1][:, 1:])
         W[8] = 0.5 * (bf[2][:, :, :-1] + bf[2][:, :, 1:])
     return Problem("random_smooth", tuple(n), xlim, tuple(bc), gamma, W, bf=bf, mhd=mhd, cfl=cfl)
+
+
+def ragged_ensemble(count=48, sizes=(32, 48, 64, 96, 128), seed=2026, gamma_range=(1.3, 1.7),
+                    cfl=0.4):
+    """Ragged ensemble of independent periodic turbulent boxes: edge sizes drawn
+    uniformly from `sizes` (cubed), gamma uniform in gamma_range, each instance
+    with its own seed (1..count)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice(sizes))
+        g = float(rng.uniform(*gamma_range))
+        p = turbulent_box(n=(n, n, n), seed=i + 1, gamma=g, cfl=cfl)
+        p.name = f"ensemble[{i}]"
+        out.append(p)
+    return out
+
+
+def acoustic_box(n=(64, 64, 64), gamma=5.0 / 3.0, amp=1.0e-3, cfl=0.4):
+    """Smooth acoustic state in a periodic unit box: rho = 1 + amp sin(2 pi x)
+    cos(2 pi y), p = 1/gamma (sound speed 1), v_x = c_s (rho - 1)/rho_0 (the
+    right-moving linear sound wave's velocity), v_y = v_z = 0, Ye passive."""
+    xlim = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    (x, y, z), _ = _centres(n, xlim)
+    rho = 1.0 + amp * np.sin(2.0 * np.pi * x)[:, None, None] * np.cos(2.0 * np.pi * y)[None, :, None] \
+        * np.ones(n[2])[None, None, :]
+    p = np.full(tuple(n), 1.0 / gamma)
+    W = np.zeros((6,) + tuple(n))
+    W[0] = rho
+    W[1] = 1.0 * (rho - 1.0)
+    W[4] = p / ((gamma - 1.0) * rho)
+    W[5] = YE_PASSIVE
+    return Problem("acoustic_box", tuple(n), xlim, (PERIODIC,) * 6, gamma, W, cfl=cfl)
+
+
+def prim_from_rho_p(rho, p, v, ye, gamma):
+    """W = (rho, v1, v2, v3, eint, Ye) from density, pressure, velocity (3, ...)."""
+    return np.stack([rho, v[0], v[1], v[2], p / ((gamma - 1.0) * rho), ye])
+
+
+def cfl_dt(W, gamma, dx, cfl):
+    """CFL step of a state, sound speed only (src/prj_timeint.c:2163-2182):
+    cfl / max over cells of sum_d (|v_d| + c_s) / dx_d."""
+    rho = W[0]
+    p = (gamma - 1.0) * rho * W[4]
+    cs = np.sqrt(gamma * p / rho)
+    s = sum((np.abs(W[1 + d]) + cs) / dx[d] for d in range(3))
+    return cfl / float(np.max(s))
+
+
+def subbox_mask(n, xlim, lo=0.25, hi=0.5):
+    """Cells whose centres lie in [lo, hi]^3."""
+    (x, y, z), _ = _centres(n, xlim)
+    mx = (x >= lo) & (x <= hi)
+    my = (y >= lo) & (y <= hi)
+    mz = (z >= lo) & (z <= hi)
+    return mx[:, None, None] & my[None, :, None] & mz[None, None, :]

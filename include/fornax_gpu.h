@@ -115,6 +115,49 @@ int fx_riemann_hlld_batch(int n, const double *WL, const double *WR, const doubl
 int fx_emf_upwind_batch(int n, const double *emf_face, const double *emf_cell,
     const double *v_norm, double *emf, void *stream);
 
+/* ---- ragged ensembles of independent hydro meshes (batched stage) ------- *
+ * No counterpart in the reference, which runs one problem per process: an
+ * ensemble advances n independent periodic/outflow/reflect hydro meshes of
+ * different sizes and gammas together, each with its own CFL dt.  Schedule
+ * mode 0 launches every instance's kernels separately (the per-instance
+ * baseline); mode 1 runs each stage as one batched launch over a table of
+ * (instance, block) entries -- one workgroup per entry, or persistent_wgs
+ * persistent workgroups draining the table through atomic queues (per XCD when
+ * xcd != 0); table 0 is instance-major, 1 interleaves the instances.  With the
+ * graph on, a whole step is captured once into a hipGraph and replayed. */
+typedef struct fx_ensemble fx_ensemble;
+int fx_ensemble_create(int n, const fx_mesh_desc *descs, fx_ensemble **out);
+int fx_ensemble_destroy(fx_ensemble *e);
+int fx_ensemble_set_prim(fx_ensemble *e, int inst, const double *W);
+int fx_ensemble_get_prim(fx_ensemble *e, int inst, int slot, double *W);
+int fx_ensemble_set_schedule(fx_ensemble *e, int mode, int persistent_wgs, int xcd, int table);
+int fx_ensemble_set_graph(fx_ensemble *e, int on);
+/* One RK2 step of every instance (CFL dt on the device, growth limit, two
+ * stages); does not synchronise. */
+int fx_ensemble_step_async(fx_ensemble *e, double cfl, double dt_factor);
+int fx_ensemble_sync(fx_ensemble *e);
+int fx_ensemble_last_dt(fx_ensemble *e, double *dt);
+int fx_ensemble_info(fx_ensemble *e, int *nblocks, long long *ncells);
+/* Times `reps` steps of each candidate schedule (advancing the state) and
+ * keeps the fastest; out[4] = mode, persistent_wgs, xcd, table. */
+int fx_ensemble_autotune(fx_ensemble *e, double cfl, double dt_factor, int reps, int *out,
+    double *best_ms);
+
+/* ---- differentiable scheme and its discrete adjoint ---------------------- *
+ * No counterpart in the reference (it has neither the van Albada limiter nor
+ * HLLE): PLM with the smooth van Albada slope (regularisation va_eps2) + HLLE
+ * with Davis speeds, RK2 with a fixed dt, periodic box, ideal gas.  Device
+ * arrays in the compact layout W[var][X][Y][Z] (6 prims, Z fastest).
+ * fx_adj_forward keeps a checkpoint every checkpoint_every steps;
+ * fx_adj_backward maps the cotangent of W_T to that of W_0 for the last
+ * forward, recomputing one segment at a time.  Stream-ordered on `stream`. */
+typedef struct fx_adj fx_adj;
+int fx_adj_create(const int *n, const double *xlim, double gamma, double va_eps2,
+    int checkpoint_every, fx_adj **out);
+int fx_adj_destroy(fx_adj *a);
+int fx_adj_forward(fx_adj *a, const double *W0, double *WT, int nsteps, double dt, void *stream);
+int fx_adj_backward(fx_adj *a, const double *gWT, double *gW0, void *stream);
+
 const char *fx_last_error(void);
 /* Library build string (arch, flags). */
 const char *fx_version(void);

@@ -5,6 +5,8 @@ the ``cpu_baseline`` leg of bench.py -- never by the product package.
 
 * :class:`OracleSim` drives ``oracle/liboracle_fx.so``, the C restatement of the
   reference RK2 stage (oracle/fx_oracle.c).
+* ``adj_*`` functions drive the differentiable van Albada/HLLE scheme and its
+  tangent-linear model (oracle/fxa_oracle.c).
 * :class:`RefSim` drives ``oracle/_ref/libprj_ref_{hydro,mhd}.so``, the serial
   build of the reference sources with ``oracle/ref_harness.c`` as its driver.
 
@@ -68,6 +70,13 @@ def oracle_lib():
         lib.fxo_hlld.argtypes = [_dp, _dp] + [C.c_double] * 5 + [_dp, _dp, _dp, _dp]
         lib.fxo_emf_upwind.argtypes = [_dp, _dp, _dp, _dp]
         lib.fxo_last_error.restype = C.c_char_p
+        lib.fxa_va.restype = C.c_double
+        lib.fxa_va.argtypes = [C.c_double, C.c_double, C.c_double]
+        lib.fxa_hlle.argtypes = [_dp, _dp, C.c_double, _dp]
+        lib.fxa_forward.argtypes = [_ip, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                    _dp]
+        lib.fxa_tlm.argtypes = [_ip, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                _dp, _dp]
         _oracle_lib = lib
     return _oracle_lib
 
@@ -252,3 +261,47 @@ class RefSim:
 
     def close(self):
         self.lib.refh_destroy()
+
+
+# ---- differentiable scheme (van Albada + HLLE, fixed dt, periodic) ----------
+
+def adj_va(sl, sr, eps2):
+    return oracle_lib().fxa_va(float(sl), float(sr), float(eps2))
+
+
+def adj_hlle(qL, qR, gamma):
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    F = np.empty(6)
+    oracle_lib().fxa_hlle(_ptr(qL), _ptr(qR), float(gamma), _ptr(F))
+    return F
+
+
+def _adj_args(n, xlim):
+    return (C.c_int * 3)(*[int(v) for v in n]), (C.c_double * 6)(*[float(v) for v in xlim])
+
+
+def adj_forward(W0, xlim, gamma, eps2, dt, nsteps):
+    """nsteps RK2 steps of the scheme from W0 [6][X][Y][Z]."""
+    W0 = np.ascontiguousarray(W0, dtype=np.float64)
+    n, xl = _adj_args(W0.shape[1:], xlim)
+    WT = np.empty_like(W0)
+    rc = oracle_lib().fxa_forward(n, xl, float(gamma), float(eps2), float(dt), int(nsteps),
+                                  _ptr(W0), _ptr(WT))
+    if rc:
+        raise MemoryError("fxa_forward")
+    return WT
+
+
+def adj_tlm(W0, dW0, xlim, gamma, eps2, dt, nsteps):
+    """(forward(W0), d forward(W0) . dW0) by the tangent-linear model."""
+    W0 = np.ascontiguousarray(W0, dtype=np.float64)
+    dW0 = np.ascontiguousarray(dW0, dtype=np.float64)
+    n, xl = _adj_args(W0.shape[1:], xlim)
+    WT = np.empty_like(W0)
+    dWT = np.empty_like(W0)
+    rc = oracle_lib().fxa_tlm(n, xl, float(gamma), float(eps2), float(dt), int(nsteps), _ptr(W0),
+                              _ptr(dW0), _ptr(WT), _ptr(dWT))
+    if rc:
+        raise MemoryError("fxa_tlm")
+    return WT, dWT

@@ -1,0 +1,150 @@
+"""Ragged ensembles of hydro meshes on the GPU.
+
+Every schedule of the batched step (per-instance launches, one batched launch
+per stage with and without the XCD remap and the interleaved table, persistent
+workgroups on atomic queues, hipGraph capture) must give, for every instance,
+bit for bit the states and dt sequence of the same problem advanced alone by
+the single-mesh path, which the other GPU tests pin to the oracle; one
+instance is also compared with the oracle directly.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from fornaxamr_amd import _native
+from fornaxamr_amd import problems as P
+from helpers import ulp_diff
+
+pytestmark = pytest.mark.gpu
+STEPS = 4
+CFL = 0.4
+FACTOR = 1.2
+
+
+def _instances():
+    return [
+        P.random_smooth(n=(32, 32, 32), seed=41, bc=(3,) * 6, gamma=1.4),
+        P.random_smooth(n=(48, 32, 64), seed=42, bc=(3, 3, 0, 0, 1, 1), gamma=1.6),
+        P.turbulent_box(n=(64, 64, 64), seed=43, gamma=1.35),
+        P.sedov(n=(32, 32, 32)),
+        P.random_smooth(n=(16, 32, 48), seed=44, bc=(0, 0, 3, 3, 1, 0), gamma=1.5),
+    ]
+
+
+@pytest.fixture(scope="module")
+def expected(gpu):
+    """Per instance: dt of each step and the state after each step, from the
+    single-mesh path."""
+    from fornaxamr_amd.mesh import GpuMesh
+    out = []
+    for pr in _instances():
+        m = GpuMesh.from_problem(pr)
+        dts, Ws = [], []
+        for _ in range(STEPS):
+            dts.append(m.step(CFL, FACTOR))
+            Ws.append(m.get_prim(0))
+        m.close()
+        out.append((dts, Ws))
+    return out
+
+
+def test_single_mesh_reference_matches_oracle(expected):
+    pr = _instances()[1]
+    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma)
+    om.set_state(pr.W)
+    dts, Ws = expected[1]
+    for s in range(STEPS):
+        assert om.step(CFL, FACTOR) == dts[s]
+    assert ulp_diff(om.W0, Ws[-1]) == 0
+
+
+SCHEDULES = [
+    (0, 0, 1, 0),       # per-instance launches
+    (1, 0, 1, 0),       # batched, XCD remap, instance-major table
+    (1, 0, 0, 0),
+    (1, 0, 1, 1),       # interleaved table
+    (1, 7, 1, 0),       # persistent, few workgroups, per-XCD queues
+    (1, 7, 0, 1),       # persistent, one queue
+    (1, 256, 1, 1),
+    (1, 600, 0, 0),     # more workgroups than table entries
+]
+
+
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_schedule_matches_single_mesh(expected, sched):
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    e = GpuEnsemble(_instances())
+    e.set_schedule(*sched)
+    for s in range(STEPS):
+        e.step_async(CFL, FACTOR)
+        dts = e.last_dt()
+        for i, (edts, _) in enumerate(expected):
+            assert dts[i] == edts[s], (sched, i, s)
+    e.sync()
+    for i, (_, Ws) in enumerate(expected):
+        assert np.array_equal(e.get_prim(i), Ws[-1]), (sched, i)
+    e.close()
+
+
+@pytest.mark.parametrize("sched", [(1, 0, 1, 0), (1, 256, 1, 1), (0, 0, 1, 0)])
+def test_graph_replays(expected, sched):
+    """A captured step replayed STEPS times, with eager work (host copies of
+    the state, an unrelated single-mesh step) between the replays; every
+    replay is checked."""
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    from fornaxamr_amd.mesh import GpuMesh
+    e = GpuEnsemble(_instances())
+    e.set_schedule(*sched)
+    e.set_graph(True)
+    other = GpuMesh.from_problem(P.random_smooth(n=(32, 32, 32), seed=9, bc=(3,) * 6))
+    for s in range(STEPS):
+        e.step_async(CFL, FACTOR)
+        e.sync()
+        dts = e.last_dt()
+        for i, (edts, Ws) in enumerate(expected):
+            assert dts[i] == edts[s], (sched, i, s)
+            assert np.array_equal(e.get_prim(i), Ws[s]), (sched, i, s)
+        other.step(0.4)
+    e.close()
+    other.close()
+
+
+def test_graph_recaptures_on_new_cfl(expected):
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    e = GpuEnsemble(_instances())
+    e.set_schedule(1, 0, 1, 0)
+    e.set_graph(True)
+    e.step_async(0.2, FACTOR)       # captured with cfl 0.2
+    e.reset()
+    for s in range(STEPS):
+        e.step_async(CFL, FACTOR)   # must not replay the cfl 0.2 graph
+    e.sync()
+    for i, (_, Ws) in enumerate(expected):
+        assert np.array_equal(e.get_prim(i), Ws[-1]), i
+    e.close()
+
+
+def test_autotune_then_step(expected):
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    e = GpuEnsemble(_instances())
+    e.set_graph(True)
+    best = e.autotune(CFL, FACTOR, reps=2)
+    assert best["mode"] == 1 and best["ms_per_step"] > 0.0
+    for _ in range(STEPS):
+        e.step_async(CFL, FACTOR)
+    e.sync()
+    for i, (_, Ws) in enumerate(expected):
+        assert np.array_equal(e.get_prim(i), Ws[-1]), i
+    e.close()
+
+
+def test_bad_arguments(gpu):
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    with pytest.raises(ValueError):
+        GpuEnsemble([P.random_smooth(n=(32, 32, 32), seed=1, mhd=True)])
+    e = GpuEnsemble(_instances()[:1])
+    with pytest.raises(_native.FxError):
+        e.set_schedule(2, 0, 0, 0)
+    assert e.lib.fx_ensemble_get_prim(e.h, 1, 0, None) == -1      # FX_EINVAL
+    assert e.lib.fx_ensemble_set_schedule(e.h, 1, -1, 0, 0) == -1
+    e.close()
