@@ -25,6 +25,13 @@ class MeshView(C.Structure):
                 ("n", C.c_int * 3), ("nh", C.c_int)]
 
 
+class BlockIO(C.Structure):
+    """fx_block_io: one block in the reference's own layouts (pointers)."""
+    _fields_ = [("W", C.c_void_p), ("var_stride", C.c_longlong), ("eosvar", C.c_void_p),
+                ("bf", C.c_void_p * 3), ("flux", C.c_void_p * 3), ("v_riemann", C.c_void_p * 3),
+                ("Bv1", C.c_void_p * 3), ("Bv2", C.c_void_p * 3)]
+
+
 class FxError(RuntimeError):
     pass
 
@@ -78,6 +85,9 @@ def load():
     lib.fx_adj_destroy.argtypes = [vp]
     lib.fx_adj_forward.argtypes = [vp, vp, vp, C.c_int, C.c_double, vp]
     lib.fx_adj_backward.argtypes = [vp, vp, vp, vp]
+    lib.fx_flux_update_blocks.argtypes = [C.c_int, vp, C.c_int, vp, vp]
+    lib.fx_flux_update_block.argtypes = [C.POINTER(BlockIO), C.c_int]
+    lib.fx_flux_div_block.argtypes = [C.POINTER(_dp), _dp, C.c_double, C.c_int, _dp]
     lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
     lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
     lib.fx_emf_upwind_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
@@ -101,5 +111,6 @@ EXPORTS = [
     "fx_ensemble_set_schedule", "fx_ensemble_set_graph", "fx_ensemble_step_async",
     "fx_ensemble_sync", "fx_ensemble_last_dt", "fx_ensemble_info", "fx_ensemble_autotune",
     "fx_adj_create", "fx_adj_destroy", "fx_adj_forward", "fx_adj_backward",
+    "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_div_block",
     "fx_last_error", "fx_version",
 ]

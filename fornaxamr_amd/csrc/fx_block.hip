@@ -1,0 +1,198 @@
+// Per-block flux assembly in the reference's own block layouts: the drop-in
+// form of prj_flux_update (src/prj_flux.c:2149-2521) and prj_flux_div
+// (src/prj_flux.c:2523-2533) for blocks kept in the reference's 20^3 storage.
+//
+// Block layouts (src/prj_defs.h:383-459, 16^3 active cells, 2 ghost layers,
+// AoSoA lane width 4):
+//   LIDX(i,j,k)        = ((i+2)*20 + (j+2))*20 + (k+2)
+//   BIDX(nc,v,i,j,k)   = ((((i+2)*20 + (j+2))*5 + (k+2)/4)*nc + v)*4 + (k+2)%4
+//   FACE_IDX(dir,...)  = LIDX with 21 slots along dir
+// Faces written (src/prj_flux.c:2221-2262): normal index 0..16 in every
+// direction; transverse 0..15 for hydro, -1..16 for MHD (CT needs one more
+// layer).  A face (i,j,k) of direction dir lies between cell (i,j,k) - e_dir
+// (left) and (i,j,k) (right).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fornax_gpu.h"
+#include "fx_kernels.hpp"
+#include "fx_physics.hpp"
+
+namespace fx {
+namespace {
+
+constexpr int NT = 256;
+constexpr int RB = 20;          // reference block edge incl. ghosts (PRJ_BS)
+constexpr int RNG = 2;          // ghost layers
+constexpr int AW = 4;           // AoSoA lane width (PRJ_AOSOA_W)
+constexpr int SPLIT = 8;        // workgroups per (block, direction)
+
+__device__ __forceinline__ int lidx(int i, int j, int k)
+{
+    return ((i + RNG) * RB + (j + RNG)) * RB + (k + RNG);
+}
+
+__device__ __forceinline__ int bidx(int nc, int v, int i, int j, int k)
+{
+    return ((((i + RNG) * RB + (j + RNG)) * (RB / AW) + (k + RNG) / AW) * nc + v) * AW +
+           (k + RNG) % AW;
+}
+
+__device__ __forceinline__ int face_idx(int dir, int i, int j, int k)
+{
+    if (dir == 0) return ((i + RNG) * RB + (j + RNG)) * RB + (k + RNG);
+    if (dir == 1) return ((i + RNG) * (RB + 1) + (j + RNG)) * RB + (k + RNG);
+    return ((i + RNG) * RB + (j + RNG)) * (RB + 1) + (k + RNG);
+}
+
+// local -> global primitive/conserved slot (src/prj_flux.c:274-349)
+__device__ __forceinline__ int gmap(int dir, int lv)
+{
+    if (lv == 0 || lv == 4 || lv == 5) return lv;
+    if (lv <= 3) return 1 + (lv - 1 + dir) % 3;
+    return 6 + (lv - 6 + dir) % 3;
+}
+
+template <int MHD>
+__device__ void block_face(const fx_block_io& b, int dir, int i, int j, int k, int* err)
+{
+    constexpr int NHV = MHD ? 9 : 6;
+    const int64_t vs = b.var_stride;
+    const double* W = b.W;
+    const int ex = dir == 0, ey = dir == 1, ez = dir == 2;
+    double WL[NHV], WR[NHV];
+#pragma unroll
+    for (int lv = 0; lv < NHV; ++lv) {
+        const double* Wv = W + (int64_t)gmap(dir, lv) * vs;
+        double q[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) q[s] = Wv[lidx(i + (s - 2) * ex, j + (s - 2) * ey, k + (s - 2) * ez)];
+        WL[lv] = q[1] + mc_half(q[0], q[1], q[2]);
+        WR[lv] = q[2] - mc_half(q[1], q[2], q[3]);
+    }
+    // pressure and gamma reconstructed from eosvar (src/prj_flux.c:555-562)
+    double pq[4], gq[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int ci = i + (s - 2) * ex, cj = j + (s - 2) * ey, ck = k + (s - 2) * ez;
+        pq[s] = b.eosvar[bidx(3, 0, ci, cj, ck)];
+        gq[s] = b.eosvar[bidx(3, 2, ci, cj, ck)];
+    }
+    const double pL = pq[1] + mc_half(pq[0], pq[1], pq[2]);
+    const double pR = pq[2] - mc_half(pq[1], pq[2], pq[3]);
+    const double gL = gq[1] + mc_half(gq[0], gq[1], gq[2]);
+    const double gR = gq[2] - mc_half(gq[1], gq[2], gq[3]);
+    double Fl[NHV], vloc[3];
+    const int c = lidx(i, j, k);
+    if (MHD) {
+        const double bn = b.bf[dir][face_idx(dir, i, j, k)];
+        WL[B1] = bn;
+        WR[B1] = bn;
+        double bv1 = 0.0, bv2 = 0.0;
+        const int rc = hlld(WL, WR, pL, pR, gL, gR, bn, Fl, vloc, &bv1, &bv2);
+        if (rc != 0) {
+            if (atomicCAS(err, 0, rc) == 0) {
+                err[1] = dir;
+                err[2] = i;
+                err[3] = j;
+                err[4] = k;
+            }
+            return;
+        }
+        b.Bv1[dir][c] = bv1;
+        b.Bv2[dir][c] = bv2;
+    } else {
+        // velocity deltas (src/prj_flux.c:1967-2036)
+        const int a1 = (dir + 1) % 3, a2 = (dir + 2) % 3;
+        const int li = i - ex, lj = j - ey, lk = k - ez;
+        const double* vn = W + (int64_t)(V1 + dir) * vs;
+        const double du = vn[c] - vn[lidx(li, lj, lk)];
+        double dtr[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int a = t == 0 ? a1 : a2;
+            const int tx = a == 0, ty = a == 1, tz = a == 2;
+            const double* va = W + (int64_t)(V1 + a) * vs;
+            const double l0 = va[lidx(li, lj, lk)];
+            const double r0 = va[c];
+            const double dl = dmin_delta(l0 - va[lidx(li - tx, lj - ty, lk - tz)],
+                                         va[lidx(li + tx, lj + ty, lk + tz)] - l0);
+            const double dr = dmin_delta(r0 - va[lidx(i - tx, j - ty, k - tz)],
+                                         va[lidx(i + tx, j + ty, k + tz)] - r0);
+            dtr[t] = dmin_delta(dl, dr);
+        }
+        hllc(WL, WR, pL, pR, gL, gR, du, dtr[0], dtr[1], Fl, vloc);
+    }
+    double* F = b.flux[dir];
+#pragma unroll
+    for (int lv = 0; lv < NHV; ++lv) F[bidx(NHV, gmap(dir, lv), i, j, k)] = Fl[lv];
+    double* vr = b.v_riemann[dir];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) vr[bidx(3, gmap(dir, 1 + r) - 1, i, j, k)] = vloc[r];
+}
+
+// grid: (block, direction, SPLIT); faces enumerated k fastest
+template <int MHD>
+__global__ void __launch_bounds__(NT) block_flux_kernel(const fx_block_io* __restrict__ blocks,
+                                                       int* err)
+{
+    const fx_block_io b = blocks[blockIdx.x];
+    const int dir = blockIdx.y;
+    const int t0 = MHD ? -1 : 0;
+    const int tn = MHD ? 18 : 16;          // transverse count
+    int lo[3], cnt[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = a == dir ? 0 : t0;
+        cnt[a] = a == dir ? 17 : tn;
+    }
+    const int nf = cnt[0] * cnt[1] * cnt[2];
+    for (int f = blockIdx.z * NT + threadIdx.x; f < nf; f += SPLIT * NT) {
+        const int k = lo[2] + f % cnt[2];
+        const int j = lo[1] + (f / cnt[2]) % cnt[1];
+        const int i = lo[0] + f / (cnt[2] * cnt[1]);
+        block_face<MHD>(b, dir, i, j, k, err);
+    }
+}
+
+// prj_flux_div over the active cells: out[v][i][j][k]
+__global__ void __launch_bounds__(NT) block_flux_div_kernel(const double* __restrict__ f0,
+                                                           const double* __restrict__ f1,
+                                                           const double* __restrict__ f2,
+                                                           double a0, double a1, double a2,
+                                                           double vol, int nvar, double* out)
+{
+    const int c = blockIdx.x * NT + threadIdx.x;
+    if (c >= 4096) return;
+    const int k = c & 15, j = (c >> 4) & 15, i = c >> 8;
+    for (int v = 0; v < nvar; ++v) {
+        out[v * 4096 + c] =
+            -(a0 * (f0[bidx(nvar, v, i + 1, j, k)] - f0[bidx(nvar, v, i, j, k)]) +
+              a1 * (f1[bidx(nvar, v, i, j + 1, k)] - f1[bidx(nvar, v, i, j, k)]) +
+              a2 * (f2[bidx(nvar, v, i, j, k + 1)] - f2[bidx(nvar, v, i, j, k)])) / vol;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_block_flux(const fx_block_io* blocks_dev, int n, int mhd, int* err,
+                             hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 grid(n, 3, SPLIT);
+    if (mhd) hipLaunchKernelGGL(block_flux_kernel<1>, grid, dim3(NT), 0, s, blocks_dev, err);
+    else hipLaunchKernelGGL(block_flux_kernel<0>, grid, dim3(NT), 0, s, blocks_dev, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_flux_div(const double* f0, const double* f1, const double* f2,
+                                 const double* area, double vol, int nvar, double* out,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(block_flux_div_kernel, dim3(4096 / NT), dim3(NT), 0, s, f0, f1, f2,
+                       area[0], area[1], area[2], vol, nvar, out);
+    return hipGetLastError();
+}
+
+}  // namespace fx

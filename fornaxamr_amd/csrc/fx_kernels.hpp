@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/fornax_gpu.h"
 #include "fx_layout.hpp"
 
 namespace fx {
@@ -108,5 +109,12 @@ hipError_t launch_adj_u2w(const AdjGeom& g, const double* W, const double* Ub, d
 // step's output in conserved form, out: that of W0; U1b 6*N and G 36*N scratch
 hipError_t launch_adj_step_reverse(const AdjGeom& g, double dt, const double* W0, const double* W1,
                                    double* U2b, double* U1b, double* G, hipStream_t s);
+
+// ---- reference block layouts (per-block drop-in) ----------------------------
+hipError_t launch_block_flux(const fx_block_io* blocks_dev, int n, int mhd, int* err,
+                             hipStream_t s);
+hipError_t launch_block_flux_div(const double* f0, const double* f1, const double* f2,
+                                 const double* area, double vol, int nvar, double* out,
+                                 hipStream_t s);
 
 }  // namespace fx

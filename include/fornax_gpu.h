@@ -115,6 +115,41 @@ int fx_riemann_hlld_batch(int n, const double *WL, const double *WR, const doubl
 int fx_emf_upwind_batch(int n, const double *emf_face, const double *emf_cell,
     const double *v_norm, double *emf, void *stream);
 
+/* ---- per-block entry points in the reference's own block layouts -------- *
+ * A block is the reference's 20^3 box (16^3 active cells + 2 ghost layers,
+ * src/prj_defs.h:383-459):
+ *   W          prims of one stage slot: variable v at W[v*var_stride + LIDX]
+ *              (W_mhd slot s: W_mhd + s*8000, var_stride = NSTAGES*8000 = 16000)
+ *   eosvar     EIDX AoSoA (P, T, gamma)
+ *   bf[3]      MHD face fields of the stage slot, FACE_IDX
+ *   flux[3]    out, VIDX AoSoA, 6 (hydro) or 9 (MHD) conserved variables
+ *   v_riemann  out, VRIDX AoSoA; Bv1/Bv2 out, IDX (MHD)
+ * Faces written are the reference's (src/prj_flux.c:2221-2262): normal index
+ * 0..16; transverse 0..15 (hydro) or -1..16 (MHD); nothing else is touched. */
+typedef struct fx_block_io {
+    const double *W;
+    long long var_stride;
+    const double *eosvar;
+    const double *bf[3];
+    double *flux[3];
+    double *v_riemann[3];
+    double *Bv1[3];
+    double *Bv2[3];
+} fx_block_io;
+/* prj_flux_update (src/prj_flux.h:4-5) over n blocks whose arrays are device
+ * memory; `blocks` is a device array of n descriptors.  One launch on
+ * `stream`, no synchronisation.  MHD: status (device int[5], zeroed by the
+ * caller) receives the first HLLD failure {code, dir, i, j, k}. */
+int fx_flux_update_blocks(int n, const fx_block_io *blocks, int mhd, int *status, void *stream);
+/* One prj_flux_update call on host arrays: copies the block in, runs the same
+ * kernel with n = 1, copies the outputs back and synchronises; an HLLD failure
+ * returns FX_EPHYS with the reference's message. */
+int fx_flux_update_block(const fx_block_io *block, int mhd);
+/* prj_flux_div (src/prj_flux.h:10) at every active cell of one block, on host
+ * VIDX flux arrays of nvar variables: fluxdiv[v][i][j][k] (16^3 per v). */
+int fx_flux_div_block(double *const *flux, const double *area, double vol, int nvar,
+    double *fluxdiv);
+
 /* ---- ragged ensembles of independent hydro meshes (batched stage) ------- *
  * No counterpart in the reference, which runs one problem per process: an
  * ensemble advances n independent periodic/outflow/reflect hydro meshes of

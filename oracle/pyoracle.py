@@ -305,3 +305,110 @@ def adj_tlm(W0, dW0, xlim, gamma, eps2, dt, nsteps):
     if rc:
         raise MemoryError("fxa_tlm")
     return WT, dWT
+
+
+# ---- one block in the reference's own layouts ---------------------------------
+
+RB = 20                  # block edge incl. ghosts
+RB_CELLS = RB ** 3
+RB_FACES = (RB + 1) * RB * RB
+
+
+def aosoa(fields):
+    """[nc][20][20][20] -> BIDX AoSoA (lane width 4) flat array."""
+    f = np.asarray(fields, dtype=np.float64)
+    nc = f.shape[0]
+    # [v][i][j][kt][kl] -> [i][j][kt][v][kl]
+    return np.ascontiguousarray(f.reshape(nc, RB, RB, RB // 4, 4).transpose(1, 2, 3, 0, 4)).ravel()
+
+
+def from_aosoa(flat, nc):
+    a = np.asarray(flat).reshape(RB, RB, RB // 4, nc, 4)
+    return np.ascontiguousarray(a.transpose(3, 0, 1, 2, 4).reshape(nc, RB, RB, RB))
+
+
+def block_inputs(seed, mhd, gamma=1.4):
+    """Deterministic smooth block arrays for the per-block flux entry: the
+    W_mhd slab [nh][2 stages][20^3], a smoothly varying gamma field (for
+    block_eosvar) and, for MHD, face fields [3][8400]."""
+    rng = np.random.default_rng(seed)
+    nh = 9 if mhd else 6
+    x = (np.arange(RB) - 1.5) / 16.0
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+
+    def smooth(amp):
+        out = np.zeros((RB, RB, RB))
+        for _ in range(4):
+            k = rng.uniform(-6.0, 6.0, 3)
+            out += np.sin(k[0] * X + k[1] * Y + k[2] * Z + rng.uniform(0, 2 * np.pi))
+        return amp * out / 4.0
+
+    slab = np.zeros((nh, 2, RB, RB, RB))
+    gam = gamma + smooth(0.05)
+    for s in range(2):
+        rho = 1.0 + smooth(0.3)
+        p = 1.0 + smooth(0.3)
+        slab[0, s] = rho
+        for d in range(3):
+            slab[1 + d, s] = smooth(0.5)
+        slab[4, s] = p / ((gam - 1.0) * rho)
+        slab[5, s] = 0.1 + smooth(0.05)
+        if mhd:
+            for d in range(3):
+                slab[6 + d, s] = 0.2 + smooth(0.3)
+    bf = None
+    if mhd:
+        bf = [0.2 + rng.uniform(-0.3, 0.3, RB_FACES) for _ in range(3)]
+    return slab, gam, bf
+
+
+def block_eosvar(slab, slot, gam):
+    """eosvar of a slot: P = (gamma - 1) rho eint, T = eint, gamma (AoSoA)."""
+    rho, e = slab[0, slot], slab[4, slot]
+    return aosoa([(gam - 1.0) * rho * e, e, gam])
+
+
+def ref_flux_update_raw(mhd, slot, slab, eosvar, bf=None):
+    """prj_flux_update of the serial reference build on one raw block."""
+    lib = ref_lib(mhd)
+    lib.refh_flux_update_raw.argtypes = [C.c_int] + [_dp] * 17
+    nh = 9 if mhd else 6
+    rc = lib.refh_init(1, 1, 1, _ptr(np.array([0.0, 1.0, 0.0, 1.0, 0.0, 1.0])),
+                       (C.c_int * 6)(*([0] * 6)), 1.4, 0.0)
+    if rc:
+        raise RuntimeError(f"refh_init failed ({rc})")
+    slab = np.ascontiguousarray(slab, dtype=np.float64)
+    eosvar = np.ascontiguousarray(eosvar, dtype=np.float64)
+    flux = [np.empty(nh * RB_CELLS) for _ in range(3)]
+    vr = [np.empty(3 * RB_CELLS) for _ in range(3)]
+    bv1 = [np.empty(RB_CELLS) for _ in range(3)]
+    bv2 = [np.empty(RB_CELLS) for _ in range(3)]
+    bfa = [np.ascontiguousarray(b, dtype=np.float64) for b in bf] if mhd else \
+        [np.zeros(RB_FACES) for _ in range(3)]
+    rc = lib.refh_flux_update_raw(int(slot), _ptr(slab), _ptr(eosvar), *[_ptr(b) for b in bfa],
+                                  *[_ptr(a) for a in flux], *[_ptr(a) for a in vr],
+                                  *[_ptr(a) for a in bv1], *[_ptr(a) for a in bv2])
+    if rc:
+        raise RuntimeError(f"refh_flux_update_raw failed ({rc})")
+    out = {"flux": flux, "v_riemann": vr}
+    if mhd:
+        out["Bv1"] = bv1
+        out["Bv2"] = bv2
+    return out
+
+
+def ref_flux_div_raw(mhd, flux):
+    """prj_flux_div at every active cell of a 1-block unit-cube mesh."""
+    lib = ref_lib(mhd)
+    lib.refh_flux_div_raw.argtypes = [_dp] * 4
+    lib.refh_block_geometry.argtypes = [_dp, _dp]
+    nh = 9 if mhd else 6
+    lib.refh_init(1, 1, 1, _ptr(np.array([0.0, 1.0, 0.0, 1.0, 0.0, 1.0])),
+                  (C.c_int * 6)(*([0] * 6)), 1.4, 0.0)
+    f = [np.ascontiguousarray(a, dtype=np.float64) for a in flux]
+    out = np.empty((nh, 16, 16, 16))
+    lib.refh_flux_div_raw(*[_ptr(a) for a in f], _ptr(out))
+    area = np.empty(3)
+    vol = np.empty(1)
+    lib.refh_block_geometry(_ptr(area), _ptr(vol))
+    return out, area, float(vol[0])

@@ -408,3 +408,97 @@ double refh_emf_upwind(int dir, const double *emf_face, const double *emf_cell,
     return prj_mhd_emf_upwind(b, dir, 0, 0, 0, emf_face, emf_cell, v_norm, 0, 0.0);
 }
 #endif
+
+/* prj_flux_update (src/prj_flux.c:2149-2521) on raw block arrays.  Inputs in
+ * the reference's own block layouts: W_slab the whole W_mhd slab
+ * (NHYDRO x NSTAGES x NCELLS, WIDX), eosvar (EIDX), bf[d] the face fields of
+ * `slot` (FACE_IDX, MHD).  They are copied into block 0 of the mesh, the
+ * output arrays of that block are filled with NaN, prj_flux_update runs for
+ * `slot`, and flux[d] (VIDX), v_riemann[d] (VRIDX) and, for MHD, Bv1[d]/Bv2[d]
+ * (IDX) are copied out whole, so untouched entries stay NaN. */
+int refh_flux_update_raw(int slot, const double *W_slab, const double *eosvar,
+    const double *bfx, const double *bfy, const double *bfz,
+    double *flux0, double *flux1, double *flux2,
+    double *vr0, double *vr1, double *vr2,
+    double *bv1_0, double *bv1_1, double *bv1_2,
+    double *bv2_0, double *bv2_1, double *bv2_2)
+{
+    prj_block *b;
+    double *fl[3] = {flux0, flux1, flux2};
+    double *vr[3] = {vr0, vr1, vr2};
+    size_t n;
+    int d;
+    (void)bfx; (void)bfy; (void)bfz;
+    (void)bv1_0; (void)bv1_1; (void)bv1_2;
+    (void)bv2_0; (void)bv2_1; (void)bv2_2;
+    if (!g_ready || slot < 0 || slot >= PRJ_BLOCK_NSTAGES) return 1;
+    b = &g_sim.mesh.blocks[0];
+    memcpy(b->W_mhd, W_slab, sizeof(double) * (size_t)PRJ_NHYDRO * PRJ_BLOCK_NSTAGES *
+        PRJ_BLOCK_NCELLS);
+    memcpy(b->eosvar, eosvar, sizeof(double) * (size_t)PRJ_NVAR_EOSVAR * PRJ_BLOCK_NCELLS);
+#if PRJ_MHD
+    {
+        const double *bf[3] = {bfx, bfy, bfz};
+        double *bvo1[3] = {bv1_0, bv1_1, bv1_2};
+        double *bvo2[3] = {bv2_0, bv2_1, bv2_2};
+        for (d = 0; d < 3; ++d) {
+            memcpy(prj_block_bf_stage(b, d, slot), bf[d], sizeof(double) * PRJ_BLOCK_NFACES);
+            for (n = 0; n < (size_t)PRJ_BLOCK_NCELLS; ++n) {
+                b->Bv1[d][n] = NAN;
+                b->Bv2[d][n] = NAN;
+            }
+        }
+#endif
+        for (d = 0; d < 3; ++d) {
+            for (n = 0; n < (size_t)PRJ_NVAR_CONS * PRJ_BLOCK_NCELLS; ++n) b->flux[d][n] = NAN;
+            for (n = 0; n < (size_t)3 * PRJ_BLOCK_NCELLS; ++n) b->v_riemann[d][n] = NAN;
+        }
+        prj_flux_update(&g_sim.eos, &g_sim.rad, &g_sim.mesh, b, prj_block_prim_stage(b, slot),
+            b->eosvar, b->flux, slot);
+        for (d = 0; d < 3; ++d) {
+            memcpy(fl[d], b->flux[d], sizeof(double) * (size_t)PRJ_NVAR_CONS * PRJ_BLOCK_NCELLS);
+            memcpy(vr[d], b->v_riemann[d], sizeof(double) * 3 * (size_t)PRJ_BLOCK_NCELLS);
+#if PRJ_MHD
+            memcpy(bvo1[d], b->Bv1[d], sizeof(double) * (size_t)PRJ_BLOCK_NCELLS);
+            memcpy(bvo2[d], b->Bv2[d], sizeof(double) * (size_t)PRJ_BLOCK_NCELLS);
+#endif
+        }
+#if PRJ_MHD
+    }
+#endif
+    return 0;
+}
+
+/* prj_flux_div (src/prj_flux.c:2523-2533) at every active cell of block 0,
+ * on flux arrays given in VIDX layout: out[v][i][j][k]. */
+int refh_flux_div_raw(const double *flux0, const double *flux1, const double *flux2,
+    double *out)
+{
+    prj_block *b;
+    double *fl[3];
+    int i, j, k, v;
+    if (!g_ready) return 1;
+    b = &g_sim.mesh.blocks[0];
+    fl[0] = (double *)flux0;
+    fl[1] = (double *)flux1;
+    fl[2] = (double *)flux2;
+    for (i = 0; i < PRJ_BLOCK_SIZE; ++i)
+    for (j = 0; j < PRJ_BLOCK_SIZE; ++j)
+    for (k = 0; k < PRJ_BLOCK_SIZE; ++k) {
+        double fd[PRJ_NVAR_CONS];
+        prj_flux_div(fl, b->area, b->vol, i, j, k, fd);
+        for (v = 0; v < PRJ_NVAR_CONS; ++v)
+            out[((size_t)v * PRJ_BLOCK_SIZE + i) * PRJ_BLOCK_SIZE * PRJ_BLOCK_SIZE +
+                (size_t)j * PRJ_BLOCK_SIZE + k] = fd[v];
+    }
+    return 0;
+}
+
+void refh_block_geometry(double *area, double *vol)
+{
+    const prj_block *b = &g_sim.mesh.blocks[0];
+    area[0] = b->area[0];
+    area[1] = b->area[1];
+    area[2] = b->area[2];
+    *vol = b->vol;
+}

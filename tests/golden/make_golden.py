@@ -5,9 +5,11 @@ oracle/Makefile compiles from the reference sources).  Fixtures are data:
 per-face / per-edge input/output vectors of the reference's own
 prj_riemann_hllc, prj_riemann_hlld and prj_mhd_emf_upwind, and, for whole
 RK2 trajectories, the SHA-256 of the input state and of the state after every
-step plus the dt sequence and a few sampled values.
+step plus the dt sequence and a few sampled values; for the per-block entry
+points, the SHA-256 of the reference's prj_flux_update / prj_flux_div outputs
+on seeded raw blocks.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--only-block]
 """
 from __future__ import annotations
 
@@ -156,7 +158,34 @@ def trajectory(name):
     return rec
 
 
+BLOCK_CASES = {"hydro": (0, 5, 1), "mhd": (1, 6, 1), "mhd_slot0": (1, 7, 0)}
+
+
+def block_flux():
+    """prj_flux_update and prj_flux_div of the reference on one raw block per
+    case (inputs from pyoracle.block_inputs(seed, mhd)): SHA-256 of every
+    output array (untouched entries NaN) and a few sampled values."""
+    recs = {}
+    for name, (mhd, seed, slot) in BLOCK_CASES.items():
+        slab, gam, bf = O.block_inputs(seed, mhd)
+        eos = O.block_eosvar(slab, slot, gam)
+        out = O.ref_flux_update_raw(mhd, slot, slab, eos, bf)
+        fdiv, area, vol = O.ref_flux_div_raw(mhd, out["flux"])
+        rec = dict(mhd=mhd, seed=seed, slot=slot, input_sha=sha(slab), eosvar_sha=sha(eos),
+                   bf_sha=[sha(b) for b in bf] if mhd else None,
+                   out_sha={k: [sha(a) for a in v] for k, v in out.items()},
+                   flux_div_sha=sha(fdiv), area=list(map(float, area)), vol=vol,
+                   samples=[[int(i), float(out["flux"][0][i])]
+                            for i in np.flatnonzero(np.isfinite(out["flux"][0]))[::1500]])
+        recs[name] = rec
+    return recs
+
+
 def main():
+    if "--only-block" in sys.argv:
+        with open(os.path.join(HERE, "block_flux.json"), "w") as f:
+            json.dump(block_flux(), f, indent=1)
+        return
     rng = np.random.default_rng(20261202)
     libh = O.ref_lib(False)
     libm = O.ref_lib(True)
@@ -168,6 +197,8 @@ def main():
     recs = {name: trajectory(name) for name in TRAJECTORIES}
     with open(os.path.join(HERE, "trajectories.json"), "w") as f:
         json.dump(recs, f, indent=1)
+    with open(os.path.join(HERE, "block_flux.json"), "w") as f:
+        json.dump(block_flux(), f, indent=1)
     print("wrote", sorted(os.listdir(HERE)))
 
 

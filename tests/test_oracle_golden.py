@@ -82,3 +82,25 @@ def test_trajectory(name):
         assert sha(sim.W0) == rec["W_sha"][s], f"step {s}"
         if pr.mhd:
             assert [sha(b) for b in sim.bf0] == rec["bf_sha"][s]
+
+
+def test_block_flux_fixture_matches_reference_build():
+    """The per-block fixture (SHA-256 of prj_flux_update / prj_flux_div outputs
+    on seeded raw blocks) reproduces from the serial reference build."""
+    import hashlib
+    import json
+    if not O.ref_available():
+        pytest.skip("serial reference build not present")
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "block_flux.json")))
+
+    def sha(a):
+        return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+    for name, g in gold.items():
+        slab, gam, bf = O.block_inputs(g["seed"], g["mhd"])
+        eos = O.block_eosvar(slab, g["slot"], gam)
+        assert sha(slab) == g["input_sha"], name
+        out = O.ref_flux_update_raw(g["mhd"], g["slot"], slab, eos, bf)
+        assert {k: [sha(a) for a in v] for k, v in out.items()} == g["out_sha"], name
+        fd, _, _ = O.ref_flux_div_raw(g["mhd"], out["flux"])
+        assert sha(fd) == g["flux_div_sha"], name
