@@ -46,14 +46,6 @@ __device__ __forceinline__ int face_idx(int dir, int i, int j, int k)
     return ((i + RNG) * RB + (j + RNG)) * (RB + 1) + (k + RNG);
 }
 
-// local -> global primitive/conserved slot (src/prj_flux.c:274-349)
-__device__ __forceinline__ int gmap(int dir, int lv)
-{
-    if (lv == 0 || lv == 4 || lv == 5) return lv;
-    if (lv <= 3) return 1 + (lv - 1 + dir) % 3;
-    return 6 + (lv - 6 + dir) % 3;
-}
-
 template <int MHD>
 __device__ void block_face(const fx_block_io& b, int dir, int i, int j, int k, int* err)
 {

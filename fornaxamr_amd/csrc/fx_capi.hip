@@ -478,6 +478,8 @@ struct fx_ensemble {
     EnsInst* d_inst = nullptr;
     EnsBlock* d_blocks[2] = {nullptr, nullptr};
     int* d_queues = nullptr;
+    int2* d_fill = nullptr;              // ghost-fill chunks (instance, first shell id)
+    int nfill = 0;
     double* dev = nullptr;               // 4 doubles per instance
     int64_t total_shell = 0;
     int nblocks = 0;
@@ -529,7 +531,7 @@ int ens_enqueue_step(fx_ensemble* e, double cfl, double factor)
     const EnsBlock* tbl = e->d_blocks[e->table];
     FX_HIP(launch_ens_dt(e->d_inst, e->n, tbl, e->nblocks, cfl, factor, s));
     for (int st = 1; st <= 2; ++st) {
-        FX_HIP(launch_ens_fill(e->d_inst, e->n, e->total_shell, st == 1 ? 0 : 1, s));
+        FX_HIP(launch_ens_fill(e->d_inst, e->d_fill, e->nfill, st == 1 ? 0 : 1, s));
         FX_HIP(launch_ens_stage(e->d_inst, tbl, e->nblocks, st, e->persistent_wgs, e->xcd, e->xcd,
                                 e->d_queues, s));
     }
@@ -553,6 +555,7 @@ int fx_ensemble_destroy(fx_ensemble* e)
     (void)hipFree(e->d_blocks[0]);
     (void)hipFree(e->d_blocks[1]);
     (void)hipFree(e->d_queues);
+    (void)hipFree(e->d_fill);
     (void)hipFree(e->dev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -633,6 +636,16 @@ int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
         if (err == hipSuccess)
             err = hipMemcpy(e->d_blocks[t], e->order[t].data(), sizeof(EnsBlock) * e->nblocks,
                             hipMemcpyHostToDevice);
+    }
+    {
+        std::vector<int2> chunks;
+        for (int i = 0; i < n; ++i)
+            for (int64_t f = 0; f < e->host_inst[i].shell_count; f += ENS_FILL_CHUNK)
+                chunks.push_back(make_int2(i, (int)f));
+        e->nfill = (int)chunks.size();
+        if (err == hipSuccess && e->nfill > 0) err = hipMalloc(&e->d_fill, sizeof(int2) * chunks.size());
+        if (err == hipSuccess && e->nfill > 0)
+            err = hipMemcpy(e->d_fill, chunks.data(), sizeof(int2) * chunks.size(), hipMemcpyHostToDevice);
     }
     if (err == hipSuccess) err = hipMalloc(&e->d_queues, 64 * sizeof(int));
     if (err == hipSuccess) err = hipMemset(e->d_queues, 0, 64 * sizeof(int));

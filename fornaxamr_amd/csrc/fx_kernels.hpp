@@ -48,7 +48,10 @@ struct EnsBlock {
 // number of ghost-shell cells of one mesh (the domain-boundary fill)
 int64_t shell_cell_count(const Geom& g);
 
-hipError_t launch_ens_fill(const EnsInst* inst, int ninst, int64_t total_shell, int slot,
+// shell cells filled by one workgroup of the batched ghost fill
+constexpr int ENS_FILL_CHUNK = 1024;
+// chunks[w] = (instance, first shell id) of workgroup w
+hipError_t launch_ens_fill(const EnsInst* inst, const int2* chunks, int nchunks, int slot,
                            hipStream_t s);
 hipError_t launch_ens_dt(const EnsInst* inst, int ninst, const EnsBlock* blocks, int nblocks,
                          double cfl, double factor, hipStream_t s);

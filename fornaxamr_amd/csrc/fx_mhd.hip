@@ -95,43 +95,34 @@ __device__ __forceinline__ void cons2prim9(const double* u, const double* b, dou
     w[8] = b[2];
 }
 
-// one HLLD face from 4-cell stencils along the normal: q[s][gv] for cells
-// c-2..c+1 in the global variable order, pressure p[s]; writes the
-// global-order flux (first 6 variables) and the face EMF inputs.  The
-// rotation to the face frame (src/prj_flux.c:274-349) is done with selects so
-// that no register array is indexed at run time.
+// one HLLD face from 4-cell stencils along the normal; the stencil of
+// variable v is rows(v)[c + (s - 2) * S], s = 0..3 (cells c-2S .. c+S, the face
+// between c - S and c), pq the pressure of those cells.  Writes the global-order flux of the
+// hydro variables and the face EMF inputs.  The rotation to the face frame
+// (src/prj_flux.c:274-349) picks the source row per variable.
 __device__ __forceinline__ double sel3(int dir, double a, double b, double c)
 {
     return dir == 0 ? a : (dir == 1 ? b : c);
 }
 
-__device__ __forceinline__ int hlld_face(int dir, const double (*q)[9], const double* p,
-                                         double gam, double bn, double* F6, double* fd3)
+template <typename Rows>
+__device__ __forceinline__ int hlld_face(int dir, const Rows& rows, int c, int S,
+                                         const double* pq, double gam, double bn, double* F6,
+                                         double* fd3)
 {
     double WL[9], WR[9];
 #pragma unroll
     for (int lv = 0; lv < 9; ++lv) {
         if (lv == B1) continue;   // the normal field is the face value
-        double s[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const double* r = q[c];
-            switch (lv) {
-            case 1: s[c] = sel3(dir, r[1], r[2], r[3]); break;
-            case 2: s[c] = sel3(dir, r[2], r[3], r[1]); break;
-            case 3: s[c] = sel3(dir, r[3], r[1], r[2]); break;
-            case 7: s[c] = sel3(dir, r[7], r[8], r[6]); break;
-            case 8: s[c] = sel3(dir, r[8], r[6], r[7]); break;
-            default: s[c] = r[lv]; break;
-            }
-        }
-        WL[lv] = s[1] + mc_half(s[0], s[1], s[2]);
-        WR[lv] = s[2] - mc_half(s[1], s[2], s[3]);
+        const double* row = rows(gmap(dir, lv));
+        const double s0 = row[c - 2 * S], s1 = row[c - S], s2 = row[c], s3 = row[c + S];
+        WL[lv] = s1 + mc_half(s0, s1, s2);
+        WR[lv] = s2 - mc_half(s1, s2, s3);
     }
     WL[B1] = bn;
     WR[B1] = bn;
-    double pL = p[1] + mc_half(p[0], p[1], p[2]);
-    double pR = p[2] - mc_half(p[1], p[2], p[3]);
+    double pL = pq[1] + mc_half(pq[0], pq[1], pq[2]);
+    double pR = pq[2] - mc_half(pq[1], pq[2], pq[3]);
     double Fl[9], vf[3], bv1 = 0.0, bv2 = 0.0;
     int rc = hlld(WL, WR, pL, pR, gam, gam, bn, Fl, vf, &bv1, &bv2);
     // global order of the hydro variables (src/prj_flux.c:609-690)
@@ -156,17 +147,29 @@ __device__ __forceinline__ void store_fdat(const MhdStageArgs& a, int dir, int x
 }
 
 // ---------------------------------------------------------------------------
+// Flux kernel.  LDS holds two full planes (x = i and i+1, 20x20 cells with the
+// 2-cell halo, 9 primitives + pressure) so that a thread keeps little state in
+// registers across the HLLD solves: the left state of its next x-face and the
+// flux through its current x-face.  Per plane each thread solves the low-side
+// y- and z-face of its own cell and the x-face i+1 (cells i..i+2 along its
+// column); half of one wave (rotating) solves the 32 high-side y/z faces on
+// the block boundary.  The high-side y/z fluxes of a cell come from the
+// neighbouring lanes (shuffles), across wave rows and from the extra faces
+// through a small LDS buffer.
 template <int STAGE>
-__global__ void __launch_bounds__(NT) mhd_flux_kernel(MhdStageArgs a)
+__global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
 {
-    __shared__ double pl[NQ][PC];
-    __shared__ double exy[6][EX];
-    __shared__ double exz[6][EX];
+    __shared__ double pl[2][NQ][PC];
+    // y-face fluxes that cross a wave boundary: rows 4, 8, 12 (first rows of
+    // waves 1-3) and 16 (the block's high-side faces); z-face fluxes at k = 16
+    __shared__ double fyb[6][5][BS];
+    __shared__ double fz16[6][BS];
 
     const Geom& g = a.g;
     const int t = threadIdx.x;
     const int j = t >> 4;
     const int k = t & 15;
+    const int lane = t & 63;
     const int b = block_of_workgroup(blockIdx.x, gridDim.x);
     const int bz = b % g.nbz;
     const int by = (b / g.nbz) % g.nby;
@@ -196,40 +199,40 @@ __global__ void __launch_bounds__(NT) mhd_flux_kernel(MhdStageArgs a)
     auto own_at = [&](int p) { return col_own + (int64_t)p * g.sx; };
     auto halo_at = [&](int p) { return col_halo + (int64_t)p * g.sx; };
 
-    // column window: q[0..8] prims, q[9] pressure
-    double cm[NQ], c0[NQ], cp[NQ], Ls[NQ], Fx[6];
-    double hpf[9];   // prefetched halo cell of plane i+1
-
+    // own cell of plane p: 9 primitives and the pressure
     auto load_own = [&](int p, double* c) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) c[q] = a.Win[q * vs + own_at(p)];
         c[9] = gm1 * c[0] * c[4];
     };
-    auto load_halo = [&](int p) {
-        if (has_halo) {
+    // plane p into LDS slot s: the own cell given, the halo cell loaded here
+    auto store_plane = [&](int s, const double* own, int p) {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) hpf[q] = a.Win[q * vs + halo_at(p)];
+        for (int q = 0; q < NQ; ++q) pl[s][q][oidx] = own[q];
+        if (has_halo) {
+            double h[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) h[q] = a.Win[q * vs + halo_at(p)];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) pl[s][q][hidx] = h[q];
+            pl[s][9][hidx] = gm1 * h[0] * h[4];
         }
     };
-    auto store_plane = [&](const double* own) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) pl[q][oidx] = own[q];
-        if (has_halo) {
-#pragma unroll
-            for (int q = 0; q < 9; ++q) pl[q][hidx] = hpf[q];
-            pl[9][hidx] = gm1 * hpf[0] * hpf[4];
-        }
-    };
-    // x-face between column cells L (cm) and R (c0) with right stencil cell cp;
-    // Ls holds the left state on entry and the next face's left state on exit.
-    auto x_face = [&](int xf, double* F6) {
-        double WL[9], WR[9], pL, pR;
+
+    double Ls[NQ];   // left state of the next x-face (rho .. b3, p)
+    double Fx[6];    // flux through the current x-face
+
+    // x-face between own cells L = plane l and R = plane l+1 with stencil
+    // cells l-1 (cm), l (c0), l+1 (c1), l+2 (c2); Ls holds the left state on
+    // entry and the left state of the next face on exit
+    auto x_face = [&](int xf, const double* c0, const double* c1, const double* c2, double* F6) {
+        double WL[9], WR[9], pL = 0.0, pR = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            double h = mc_half(cm[q], c0[q], cp[q]);
-            double r = c0[q] - h;
+            double h = mc_half(c0[q], c1[q], c2[q]);
+            double r = c1[q] - h;
             if (q < 9) { WL[q] = Ls[q]; WR[q] = r; } else { pL = Ls[9]; pR = r; }
-            Ls[q] = c0[q] + h;
+            Ls[q] = c1[q] + h;
         }
         double bn = a.bf_in[cell_off(g, x0 + xf, y0 + j, z0 + k)];
         WL[B1] = bn;
@@ -243,25 +246,18 @@ __global__ void __launch_bounds__(NT) mhd_flux_kernel(MhdStageArgs a)
         store_fdat(a, 0, x0 + xf, y0 + j, z0 + k, fd3);
     };
 
-    // ---- prologue ------------------------------------------------------
+    // ---- prologue: planes 0, 1 into LDS; x-face 0 -------------------------
     {
-        double cmm[NQ];
+        double cmm[NQ], cm[NQ], c0[NQ], c1[NQ];
         load_own(-2, cmm);
         load_own(-1, cm);
         load_own(0, c0);
-        load_own(1, cp);
-        // left state of x-face 0 from cell -1
+        load_own(1, c1);
+        store_plane(0, c0, 0);
+        store_plane(1, c1, 1);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) Ls[q] = cm[q] + mc_half(cmm[q], cm[q], c0[q]);
-        x_face(0, Fx);
-        // plane 0 into LDS
-        load_halo(0);
-        // window for step 0: cm = cell 0, c0 = cell 1, cp = cell 2
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) { cm[q] = c0[q]; c0[q] = cp[q]; }
-        store_plane(cm);
-        load_own(2, cp);
-        load_halo(1);
+        x_face(0, cm, c0, c1, Fx);
         __syncthreads();
     }
 
@@ -270,16 +266,13 @@ __global__ void __launch_bounds__(NT) mhd_flux_kernel(MhdStageArgs a)
     const double* bfz = a.bf_in + 2 * g.fvstride;
 
     for (int i = 0; i < BS; ++i) {
-        double w0[9];
-        if (STAGE == 2) {
-#pragma unroll
-            for (int q = 0; q < 9; ++q) w0[q] = a.W0[q * vs + own_at(i)];
-        }
-        // y-face and z-face on the low side of the own cell; extra high-side
-        // faces by half of one wave
+        const int A = i & 1;       // LDS slot of plane i
+        const int B = A ^ 1;       // plane i + 1
+        // ---- y/z faces of plane i ------------------------------------------
         const int xw = i & 3;
-        const int lane = t & 63;
         const bool extra = (t >> 6) == xw && lane < 32;
+        double Fy[6], Fz[6];
+#pragma unroll 1
         for (int f = 0; f < 3; ++f) {
             if (f == 2 && !extra) continue;
             int dir, fj = j, fk = k;
@@ -289,70 +282,88 @@ __global__ void __launch_bounds__(NT) mhd_flux_kernel(MhdStageArgs a)
             else { dir = 2; fj = lane - 16; fk = BS; }
             const int c = (fj + NG) * PW + (fk + NG);
             const int S = dir == 1 ? PW : 1;
-            double q[4][9], p[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int cc = c + (s - 2) * S;
-#pragma unroll
-                for (int v = 0; v < 9; ++v) q[s][v] = pl[v][cc];
-                p[s] = pl[9][cc];
-            }
             const int gx = x0 + i, gy = y0 + fj, gz = z0 + fk;
             double bn = (dir == 1 ? bfy : bfz)[cell_off(g, gx, gy, gz)];
             double F6[6], fd3[3];
-            int rc = hlld_face(dir, q, p, gam, bn, F6, fd3);
+            const double pq[4] = {pl[A][9][c - 2 * S], pl[A][9][c - S], pl[A][9][c],
+                                  pl[A][9][c + S]};
+            int rc = hlld_face(dir, [&](int v) { return &pl[A][v][0]; }, c, S, pq, gam, bn, F6,
+                               fd3);
             if (rc) set_error(a.err, rc, dir, gx, gy, gz, STAGE);
             store_fdat(a, dir, gx, gy, gz, fd3);
-            if (dir == 1) {
+            if (f == 0) {
 #pragma unroll
-                for (int v = 0; v < 6; ++v) exy[v][fj * BS + fk] = F6[v];
+                for (int v = 0; v < 6; ++v) Fy[v] = F6[v];
+                if ((j & 3) == 0 && j > 0) {
+#pragma unroll
+                    for (int v = 0; v < 6; ++v) fyb[v][j >> 2][k] = F6[v];
+                }
+            } else if (f == 1) {
+#pragma unroll
+                for (int v = 0; v < 6; ++v) Fz[v] = F6[v];
+            } else if (dir == 1) {
+#pragma unroll
+                for (int v = 0; v < 6; ++v) fyb[v][4][fk] = F6[v];
             } else {
 #pragma unroll
-                for (int v = 0; v < 6; ++v) exz[v][fj * (BS + 1) + fk] = F6[v];
+                for (int v = 0; v < 6; ++v) fz16[v][fj] = F6[v];
             }
         }
+        // ---- x-face i+1 from cells i, i+1 (LDS) and i+2 (loaded) ------------
+        double c2[NQ];
+        load_own(i + 2, c2);
         double Fxn[6];
-        x_face(i + 1, Fxn);
+        {
+            double c0[NQ], c1[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                c0[q] = pl[A][q][oidx];
+                c1[q] = pl[B][q][oidx];
+            }
+            x_face(i + 1, c0, c1, c2, Fxn);
+        }
         __syncthreads();
 
-        // divergence in the reference's x, y, z order and the RK combination
+        // ---- divergence (x, y, z order) and the RK combination -------------
         double fd[6];
 #pragma unroll
         for (int v = 0; v < 6; ++v) {
-            fd[v] = -(a1 * (Fxn[v] - Fx[v]) + a2 * (exy[v][(j + 1) * BS + k] - exy[v][j * BS + k]) +
-                      a3 * (exz[v][j * (BS + 1) + k + 1] - exz[v][j * (BS + 1) + k])) / vol;
-        }
-        double u[6], u1[6];
-        if (STAGE == 1) {
-            prim2cons9(cm, u);
-#pragma unroll
-            for (int v = 0; v < 6; ++v) u1[v] = u[v] + dt * (0.0 + fd[v]);
-        } else {
-            double us[6];
-            prim2cons9(w0, u);
-            prim2cons9(cm, us);
-#pragma unroll
-            for (int v = 0; v < 6; ++v) {
-                double ut = us[v] + dt * (0.0 + fd[v]);
-                u1[v] = 0.5 * u[v] + 0.5 * ut;
-            }
+            const double fy_dn = __shfl_down(Fy[v], 16);
+            const double fz_dn = __shfl_down(Fz[v], 1);
+            const double fy_hi = (j & 3) != 3 ? fy_dn : fyb[v][(j + 1) >> 2][k];
+            const double fz_hi = k != BS - 1 ? fz_dn : fz16[v][j];
+            fd[v] = -(a1 * (Fxn[v] - Fx[v]) + a2 * (fy_hi - Fy[v]) + a3 * (fz_hi - Fz[v])) / vol;
         }
         {
+            double cm[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) cm[q] = pl[A][q][oidx];
+            double u[6], u1[6];
+            if (STAGE == 1) {
+                prim2cons9(cm, u);
+#pragma unroll
+                for (int v = 0; v < 6; ++v) u1[v] = u[v] + dt * (0.0 + fd[v]);
+            } else {
+                double w0[9], us[6];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) w0[q] = a.W0[q * vs + own_at(i)];
+                prim2cons9(w0, u);
+                prim2cons9(cm, us);
+#pragma unroll
+                for (int v = 0; v < 6; ++v) {
+                    double ut = us[v] + dt * (0.0 + fd[v]);
+                    u1[v] = 0.5 * u[v] + 0.5 * ut;
+                }
+            }
             const int64_t o = own_at(i);
 #pragma unroll
             for (int v = 0; v < 6; ++v) a.U1[v * vs + o] = u1[v];
         }
 #pragma unroll
         for (int v = 0; v < 6; ++v) Fx[v] = Fxn[v];
-        // advance: plane i+1 (= c0 + prefetched halo) into LDS
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) { cm[q] = c0[q]; c0[q] = cp[q]; }
         __syncthreads();
-        if (i < BS - 1) {
-            store_plane(cm);
-            if (i + 3 <= BS + 1) load_own(i + 3, cp);
-            load_halo(i + 2);
-        }
+        // ---- plane i+2 into the slot of plane i ----------------------------
+        if (i < BS - 1) store_plane(A, c2, i + 2);
         __syncthreads();
     }
 }
@@ -419,19 +430,18 @@ __global__ void __launch_bounds__(256) mhd_ghostface_kernel(MhdStageArgs a, int6
     if (!decode_ghostface(a.g, id, dir, x, y, z)) return;
     const Geom& g = a.g;
     const double gm1 = g.gamma - 1.0;
-    const int e[3] = {dir == 0, dir == 1, dir == 2};
-    double q[4][9], p[4];
+    const int S = dir == 0 ? (int)g.sx : (dir == 1 ? (int)g.sy : 1);
+    const double* base = a.Win + cell_off(g, x, y, z);
+    const int64_t vs = g.vstride;
+    double pq[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const int o = s - 2;
-        const int64_t off = cell_off(g, x + o * e[0], y + o * e[1], z + o * e[2]);
-#pragma unroll
-        for (int v = 0; v < 9; ++v) q[s][v] = a.Win[v * g.vstride + off];
-        p[s] = gm1 * q[s][0] * q[s][4];
+        const double* cell = base + (s - 2) * S;
+        pq[s] = gm1 * cell[0] * cell[4 * vs];
     }
     double bn = bf_clamped(g, a.bf_in, dir, x, y, z);
     double F6[6], fd3[3];
-    int rc = hlld_face(dir, q, p, g.gamma, bn, F6, fd3);
+    int rc = hlld_face(dir, [&](int v) { return base + v * vs; }, 0, S, pq, g.gamma, bn, F6, fd3);
     if (rc) set_error(a.err, rc, dir, x, y, z, stage);
     store_fdat(a, dir, x, y, z, fd3);
 }

@@ -17,6 +17,15 @@ namespace fx {
 enum { RHO = 0, V1 = 1, V2 = 2, V3 = 3, EINT = 4, YE = 5, B1 = 6, B2 = 7, B3 = 8 };
 enum { MOM1 = 1, MOM2 = 2, MOM3 = 3, ETOT = 4 };
 
+// local (face-frame) -> global primitive / conserved slot of direction dir
+// (src/prj_flux.c:274-349): cyclic permutation of the vector components
+__device__ __forceinline__ int gmap(int dir, int lv)
+{
+    if (lv == 0 || lv == 4 || lv == 5) return lv;
+    if (lv <= 3) return 1 + (lv - 1 + dir) % 3;
+    return 6 + (lv - 6 + dir) % 3;
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
@@ -288,7 +297,6 @@ __device__ __forceinline__ int hlld(const double* WL, const double* WR, double p
     double SR = dmax(L.vx + cfL, R.vx + cfR);
     if (!(SL < SR)) return HLLD_BAD_ORDER;
 
-    double UL[9], UR[9], FL[9], FR[9];
     if (0.0 <= SL || SR <= 0.0) {
         const HlldSide& s = 0.0 <= SL ? L : R;
         hlld_flux(s, bn, flux);
@@ -304,11 +312,13 @@ __device__ __forceinline__ int hlld(const double* WL, const double* WR, double p
     double ptl = L.pt + L.rho * (SL - L.vx) * (SM - L.vx);
     double ptr = R.pt + R.rho * (SR - R.vx) * (SM - R.vx);
     double pt_star = 0.5 * (ptl + ptr);
-    hlld_flux(L, bn, FL);
-    hlld_flux(R, bn, FR);
-    hlld_cons(L.rho, L.vx, L.vy, L.vz, bn, L.by, L.bz, L.ye, L.e, UL);
-    hlld_cons(R.rho, R.vx, R.vy, R.vz, bn, R.by, R.bz, R.ye, R.e, UR);
     if (pt_star <= 0.0 || !isfinite(pt_star)) {
+        // HLL fallback (src/prj_riemann.c:393-407): both outer states needed
+        double UL[9], UR[9], FL[9], FR[9];
+        hlld_flux(L, bn, FL);
+        hlld_flux(R, bn, FR);
+        hlld_cons(L.rho, L.vx, L.vy, L.vz, bn, L.by, L.bz, L.ye, L.e, UL);
+        hlld_cons(R.rho, R.vx, R.vy, R.vz, bn, R.by, R.bz, R.ye, R.e, UR);
         double inv = 1.0 / (SR - SL);
         double Uh[9];
         for (int v = 0; v < 9; ++v) {
@@ -328,8 +338,9 @@ __device__ __forceinline__ int hlld(const double* WL, const double* WR, double p
     const bool left = 0.0 <= SM;
     const HlldSide& A = left ? L : R;        // upwind outer side
     const double SA = left ? SL : SR;
-    const double* FA = left ? FL : FR;
-    const double* UA = left ? UL : UR;
+    double FA[9], UA[9];
+    hlld_flux(A, bn, FA);
+    hlld_cons(A.rho, A.vx, A.vy, A.vz, bn, A.by, A.bz, A.ye, A.e, UA);
     HlldStar As;
     if ((rc = hlld_outer(A, bn, SA, SM, pt_star, As)) != 0) return rc;
     double sqA = sqrt(As.rho);

@@ -97,18 +97,21 @@ __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const doubl
     fill_shell_cell(g, W, bf, id);
 }
 
-// batched fill over the shells of all instances of an ensemble
-__global__ void ens_fill_kernel(const EnsInst* __restrict__ inst, int ninst, int64_t total, int slot)
+// batched fill over the shells of all instances of an ensemble: workgroup w
+// fills ids [first, first + ENS_FILL_CHUNK) of the shell of instance inst,
+// (inst, first) = chunks[w], so the instance is uniform per workgroup
+__global__ void __launch_bounds__(256) ens_fill_kernel(const EnsInst* __restrict__ inst,
+                                                       const int2* __restrict__ chunks, int slot)
 {
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= total) return;
-    int lo = 0, hi = ninst - 1;
-    while (lo < hi) {                    // last instance with shell_begin <= id
-        int mid = (lo + hi + 1) >> 1;
-        if (inst[mid].shell_begin <= id) lo = mid; else hi = mid - 1;
+    const int2 c = chunks[blockIdx.x];
+    const EnsInst& I = inst[c.x];
+    const Geom g = I.g;
+    double* W = I.W[slot];
+    const int64_t end = I.shell_count;
+    for (int r = 0; r < ENS_FILL_CHUNK; r += 256) {
+        const int64_t id = (int64_t)c.y + r + threadIdx.x;
+        if (id < end) fill_shell_cell(g, W, nullptr, id);
     }
-    const EnsInst& I = inst[lo];
-    fill_shell_cell(I.g, I.W[slot], nullptr, id - I.shell_begin);
 }
 
 __global__ void dt_init_kernel(unsigned long long* s)
@@ -204,17 +207,18 @@ __global__ void __launch_bounds__(256) ens_dt_reduce_kernel(const EnsInst* __res
 {
     __shared__ double red[256];
     const EnsBlock e = blocks[blockIdx.x];
-    const EnsInst& I = inst[e.inst];
+    const Geom g = inst[e.inst].g;
+    const double* W = inst[e.inst].W[0];
     const int j = threadIdx.x >> 4, k = threadIdx.x & 15;
     double best = 1.0e99;
     for (int i = 0; i < BS; ++i) {
-        int64_t o = cell_off(I.g, e.bx * BS + i, e.by * BS + j, e.bz * BS + k);
-        double d = cell_dt(I.g, I.W[0], o, cfl);
+        int64_t o = cell_off(g, e.bx * BS + i, e.by * BS + j, e.bz * BS + k);
+        double d = cell_dt(g, W, o, cfl);
         if (d < best) best = d;
     }
     best = block_min(best, red);
     if (threadIdx.x == 0)
-        atomicMin(reinterpret_cast<unsigned long long*>(I.dev + 2),
+        atomicMin(reinterpret_cast<unsigned long long*>(inst[e.inst].dev + 2),
                   (unsigned long long)__double_as_longlong(best));
 }
 
@@ -319,11 +323,11 @@ hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* sc
     return hipGetLastError();
 }
 
-hipError_t launch_ens_fill(const EnsInst* inst, int ninst, int64_t total_shell, int slot,
+hipError_t launch_ens_fill(const EnsInst* inst, const int2* chunks, int nchunks, int slot,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(ens_fill_kernel, dim3(grid_for(total_shell, 256)), dim3(256), 0, s, inst,
-                       ninst, total_shell, slot);
+    if (nchunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ens_fill_kernel, dim3(nchunks), dim3(256), 0, s, inst, chunks, slot);
     return hipGetLastError();
 }
 
