@@ -88,6 +88,7 @@ def load():
     lib.fx_flux_update_blocks.argtypes = [C.c_int, vp, C.c_int, vp, vp]
     lib.fx_flux_update_block.argtypes = [C.POINTER(BlockIO), C.c_int]
     lib.fx_flux_div_block.argtypes = [C.POINTER(_dp), _dp, C.c_double, C.c_int, _dp]
+    lib.fx_mesh_set_integrator.argtypes = [vp, C.c_int]
     lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
     lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
     lib.fx_emf_upwind_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
@@ -112,5 +113,6 @@ EXPORTS = [
     "fx_ensemble_sync", "fx_ensemble_last_dt", "fx_ensemble_info", "fx_ensemble_autotune",
     "fx_adj_create", "fx_adj_destroy", "fx_adj_forward", "fx_adj_backward",
     "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_div_block",
+    "fx_mesh_set_integrator",
     "fx_last_error", "fx_version",
 ]

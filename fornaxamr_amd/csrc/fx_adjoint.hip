@@ -179,11 +179,12 @@ __device__ __forceinline__ void hlle_vjp(const double* qL, const double* qR, dou
     const double den = bp - bm;
     double bpb = 0.0, bmb = 0.0, bpmb = 0.0, denb = 0.0;
     double ULb[6], URb[6], FLb[6], FRb[6];
+    const double inv_den = 1.0 / den;
 #pragma unroll
     for (int v = 0; v < 6; ++v) {
         const double num = (bp * L.F[v] - bm * R.F[v]) + bpm * (R.U[v] - L.U[v]);
-        const double F = num / den;
-        const double nb = Fb[v] / den;
+        const double F = num * inv_den;
+        const double nb = Fb[v] * inv_den;
         denb -= nb * F;
         bpb += nb * L.F[v];
         bmb -= nb * R.F[v];
@@ -213,20 +214,32 @@ __device__ __forceinline__ void hlle_vjp(const double* qL, const double* qR, dou
     qRb[0] -= cRb * (cR / (2.0 * qR[0]));
 }
 
-// face states of the low face of cell c along d from cells c-2..c+1
+// face states of the low face of cell c along d from cells c-2..c+1, in the
+// face frame (rho, v_n, v_t1, v_t2, p, Ye); the cells are read in that order so
+// that no register array is indexed by the direction at run time
 __device__ __forceinline__ void face_states(const AdjGeom& g, const double* __restrict__ W, int d,
                                             int x, int y, int z, double* qL, double* qR)
 {
+    const int64_t N = g.N;
+    const int64_t o1 = (int64_t)rmap(d, 1) * N, o2 = (int64_t)rmap(d, 2) * N,
+                  o3 = (int64_t)rmap(d, 3) * N;
     double q[4][6];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) cell_q(g, W, ashift(g, x, y, z, d, s - 2), q[s]);
+    for (int s = 0; s < 4; ++s) {
+        const int64_t c = ashift(g, x, y, z, d, s - 2);
+        q[s][0] = W[c];
+        q[s][1] = W[o1 + c];
+        q[s][2] = W[o2 + c];
+        q[s][3] = W[o3 + c];
+        q[s][4] = ((g.gamma - 1.0) * W[c]) * W[4 * N + c];
+        q[s][5] = W[5 * N + c];
+    }
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
-        const int v = rmap(d, r);
-        const double dl = va(q[1][v] - q[0][v], q[2][v] - q[1][v], g.eps2);
-        const double dr = va(q[2][v] - q[1][v], q[3][v] - q[2][v], g.eps2);
-        qL[r] = q[1][v] + 0.5 * dl;
-        qR[r] = q[2][v] - 0.5 * dr;
+        const double dl = va(q[1][r] - q[0][r], q[2][r] - q[1][r], g.eps2);
+        const double dr = va(q[2][r] - q[1][r], q[3][r] - q[2][r], g.eps2);
+        qL[r] = q[1][r] + 0.5 * dl;
+        qR[r] = q[2][r] - 0.5 * dr;
     }
 }
 
@@ -377,7 +390,7 @@ __global__ void __launch_bounds__(NT) adj_u2w_kernel(int64_t N, const double* __
 // lambda_{c-1}) with lambda = Ubar, then the HLLE VJP; writes the face-state
 // cotangents in global variable order: G[(d*12 + v)*N + c] (left state),
 // G[(d*12 + 6 + v)*N + c] (right state).
-__global__ void __launch_bounds__(NT) adj_face_vjp_kernel(AdjGeom g, const double* __restrict__ W,
+__global__ void __launch_bounds__(NT, 2) adj_face_vjp_kernel(AdjGeom g, const double* __restrict__ W,
                                                          const double* __restrict__ Ub,
                                                          double coef, double* __restrict__ G)
 {

@@ -27,8 +27,11 @@ struct fx_mesh {
     double* bf[3] = {nullptr, nullptr, nullptr};   // [slot 0, slot 1, spare]
     double* U1 = nullptr;
     double* fdat = nullptr;
-    double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] reduction scratch
+    double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] reduction scratch, [3] fx_calc_dt,
+                             // [4] eSSPRK substep dt
     int* err = nullptr;      // [code, dir, x, y, z, stage]
+    int integrator = FX_RK2;
+    double* saved[2] = {nullptr, nullptr};   // eSSPRK(10,4) saved states u^n, u5
     hipStream_t stream = nullptr;
     size_t cell_bytes = 0;   // bytes of one padded variable
     // optional event timing of the stage kernels (excluding the ghost fill)
@@ -257,8 +260,8 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
         if (e == hipSuccess) e = hipMalloc(&m->U1, 6 * m->cell_bytes);
         if (e == hipSuccess) e = hipMalloc(&m->fdat, 9 * m->cell_bytes);
     }
-    if (e == hipSuccess) e = hipMalloc(&m->dev, 4 * sizeof(double));
-    if (e == hipSuccess) e = hipMemset(m->dev, 0, 4 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&m->dev, 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(m->dev, 0, 8 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&m->err, 8 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(m->err, 0, 8 * sizeof(int));
     if (e != hipSuccess) {
@@ -290,6 +293,8 @@ int fx_mesh_destroy(fx_mesh* m)
     (void)hipFree(m->U1);
     (void)hipFree(m->fdat);
     (void)hipFree(m->dev);
+    (void)hipFree(m->saved[0]);
+    (void)hipFree(m->saved[1]);
     (void)hipFree(m->err);
     delete m;
     return FX_OK;
@@ -360,11 +365,64 @@ int fx_stage(fx_mesh* m, int stage, double dt)
     return run_stage(m, stage, nullptr, dt);
 }
 
+int fx_mesh_set_integrator(fx_mesh* m, int integrator)
+{
+    if (int rc = check_mesh(m)) return rc;
+    if (integrator != FX_RK2 && integrator != FX_ESSPRK10_4)
+        return fail(FX_EINVAL, "unknown integrator");
+    if (integrator == FX_ESSPRK10_4) {
+        if (m->g.mhd) return fail(FX_EINVAL, "eSSPRK(10,4) is implemented for hydro meshes only");
+        FX_HIP(hipStreamSynchronize(m->stream));
+        for (int s = 0; s < 2; ++s) {
+            if (m->saved[s] == nullptr) {
+                hipError_t e = hipMalloc(&m->saved[s], m->cell_bytes * m->nh);
+                if (e != hipSuccess)
+                    return fail(e == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,
+                                std::string("fx_mesh_set_integrator: ") + hipGetErrorString(e));
+            }
+        }
+    }
+    m->integrator = integrator;
+    return FX_OK;
+}
+
+namespace {
+
+// One forward-Euler substep of eSSPRK: stage-1 form with the device substep
+// dt, slot 0 -> slot 1, then the slots swap (the state stays in slot 0).
+int essprk_substep(fx_mesh* m)
+{
+    if (int rc = run_stage(m, 1, m->dev + 4, 0.0)) return rc;
+    std::swap(m->W[0], m->W[1]);
+    return FX_OK;
+}
+
+// Ketcheson eSSPRK(10,4), low-storage form (src/prj_timeint.c:2661-2730)
+int essprk10_4_step(fx_mesh* m)
+{
+    const size_t bytes = m->cell_bytes * m->nh;
+    FX_HIP(launch_div_scalar(m->dev + 0, m->dev + 4, 6.0, m->stream));     // dt / 6
+    FX_HIP(hipMemcpyAsync(m->saved[0], m->W[0], bytes, hipMemcpyDeviceToDevice, m->stream));
+    for (int n = 0; n < 5; ++n)
+        if (int rc = essprk_substep(m)) return rc;
+    FX_HIP(hipMemcpyAsync(m->saved[1], m->W[0], bytes, hipMemcpyDeviceToDevice, m->stream));
+    FX_HIP(launch_blend(m->g, m->W[0], m->saved[0], 3.0 / 5.0, m->stream));
+    for (int n = 0; n < 5; ++n)
+        if (int rc = essprk_substep(m)) return rc;
+    FX_HIP(launch_blend(m->g, m->W[0], m->saved[1], 3.0 / 8.0, m->stream));
+    FX_HIP(launch_blend(m->g, m->W[0], m->saved[0], 1.0 / 25.0, m->stream));
+    return FX_OK;
+}
+
+}  // namespace
+
 int fx_step_async(fx_mesh* m, double cfl, double dt_factor)
 {
     if (int rc = check_mesh(m)) return rc;
+    const bool ssp = m->integrator == FX_ESSPRK10_4;
     FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 2, m->dev + 0, m->dev + 1, dt_factor,
-                          m->stream));
+                          m->stream, ssp ? 6.0 : 1.0));
+    if (ssp) return essprk10_4_step(m);
     if (int rc = run_stage(m, 1, m->dev, 0.0)) return rc;
     return run_stage(m, 2, m->dev, 0.0);
 }

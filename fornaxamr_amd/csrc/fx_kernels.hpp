@@ -76,8 +76,15 @@ hipError_t launch_bf2bc(const Geom& g, double* W, const double* bf, hipStream_t 
 // CFL time step: dt_out[0] = min over active cells of cfl / denom; if
 // dt_prev is non-null and *dt_prev > 0, also applies min(dt, factor * dt_prev)
 // and stores the result to *dt_prev.
+// CFL dt (times `scale`, the integrator's SSP coefficient), growth-limited
 hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* scratch,
-                          double* dt_out, double* dt_prev, double factor, hipStream_t s);
+                          double* dt_out, double* dt_prev, double factor, hipStream_t s,
+                          double scale = 1.0);
+// eSSPRK convex blend of the active cells: W <- prim(w u(Ws) + (1 - w) u(W))
+hipError_t launch_blend(const Geom& g, double* W, const double* Ws, double saved_weight,
+                        hipStream_t s);
+// dst[0] = src[0] / div on the device
+hipError_t launch_div_scalar(const double* src, double* dst, double div, hipStream_t s);
 
 // per-face / per-edge batched entry points (global-memory SoA arrays)
 hipError_t launch_hllc_batch(int n, const double* WL, const double* WR, const double* pL,

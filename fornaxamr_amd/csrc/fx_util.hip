@@ -170,11 +170,14 @@ __global__ void __launch_bounds__(256) dt_reduce_kernel(Geom g, const double* __
     }
 }
 
-// dt = min(dt_new, factor * dt_prev) after the first step (src/main.c:517-522)
+// dt = min(dt_new, factor * dt_prev) after the first step (src/main.c:517-522);
+// dt_new is the CFL minimum times the integrator's SSP scale (1 for RK2, 6 for
+// eSSPRK(10,4): src/prj_timeint.c:2246-2253)
 __device__ __forceinline__ void dt_finalize(const unsigned long long* s, double* dt_out,
-                                            double* dt_prev, double factor)
+                                            double* dt_prev, double factor, double scale = 1.0)
 {
     double dt_new = __longlong_as_double((long long)s[0]);
+    if (scale != 1.0) dt_new *= scale;
     double dt = dt_new;
     if (dt_prev != nullptr) {
         double prev = dt_prev[0];
@@ -188,9 +191,9 @@ __device__ __forceinline__ void dt_finalize(const unsigned long long* s, double*
 }
 
 __global__ void dt_finalize_kernel(const unsigned long long* s, double* dt_out, double* dt_prev,
-                                   double factor)
+                                   double factor, double scale)
 {
-    dt_finalize(s, dt_out, dt_prev, factor);
+    dt_finalize(s, dt_out, dt_prev, factor, scale);
 }
 
 __global__ void ens_dt_init_kernel(const EnsInst* __restrict__ inst, int ninst)
@@ -302,6 +305,80 @@ int64_t shell_cell_count(const Geom& g)
     return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 4 * (int64_t)g.nx * g.ny;
 }
 
+// eSSPRK convex blend of the active cells (src/prj_timeint.c:2382-2417, hydro):
+// u = w_s u(W_saved) + (1 - w_s) u(W), W <- prim(u)
+__global__ void __launch_bounds__(256) blend_kernel(Geom g, double* __restrict__ W,
+                                                    const double* __restrict__ Ws,
+                                                    double saved_weight)
+{
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int z = (int)(id % g.nz);
+    const int64_t r = id / g.nz;
+    const int y = (int)(r % g.ny);
+    const int x = (int)(r / g.ny);
+    const int64_t o = cell_off(g, x, y, z);
+    const int64_t vs = g.vstride;
+    const double current_weight = 1.0 - saved_weight;
+    double w[6], ws[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+        w[v] = W[v * vs + o];
+        ws[v] = Ws[v * vs + o];
+    }
+    double u[6], us[6];
+    u[0] = w[0];
+    u[1] = w[0] * w[1];
+    u[2] = w[0] * w[2];
+    u[3] = w[0] * w[3];
+    u[4] = w[0] * w[4] + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+    u[5] = w[0] * w[5];
+    us[0] = ws[0];
+    us[1] = ws[0] * ws[1];
+    us[2] = ws[0] * ws[2];
+    us[3] = ws[0] * ws[3];
+    us[4] = ws[0] * ws[4] + 0.5 * ws[0] * (ws[1] * ws[1] + ws[2] * ws[2] + ws[3] * ws[3]);
+    us[5] = ws[0] * ws[5];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) u[v] = saved_weight * us[v] + current_weight * u[v];
+    // cons -> prim (src/prj_eos.c:2160-2220)
+    const double rho = u[0];
+    if (rho == 0.0) {
+#pragma unroll
+        for (int v = 0; v < 6; ++v) W[v * vs + o] = 0.0;
+        return;
+    }
+    const double v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
+    const double kinetic = 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
+    W[o] = rho;
+    W[vs + o] = v1;
+    W[2 * vs + o] = v2;
+    W[3 * vs + o] = v3;
+    W[4 * vs + o] = u[4] / rho - kinetic;
+    W[5 * vs + o] = u[5] / rho;
+}
+
+__global__ void scale_kernel(const double* src, double* dst, double div)
+{
+    dst[0] = src[0] / div;
+}
+
+hipError_t launch_blend(const Geom& g, double* W, const double* Ws, double saved_weight,
+                        hipStream_t s)
+{
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(blend_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, W, Ws,
+                       saved_weight);
+    return hipGetLastError();
+}
+
+hipError_t launch_div_scalar(const double* src, double* dst, double div, hipStream_t s)
+{
+    hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, src, dst, div);
+    return hipGetLastError();
+}
+
 hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s)
 {
     const int64_t total = shell_cell_count(g);
@@ -311,7 +388,8 @@ hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipS
 }
 
 hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* scratch,
-                          double* dt_out, double* dt_prev, double factor, hipStream_t s)
+                          double* dt_out, double* dt_prev, double factor, hipStream_t s,
+                          double scale)
 {
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(scratch);
     hipLaunchKernelGGL(dt_init_kernel, dim3(1), dim3(1), 0, s, acc);
@@ -319,7 +397,8 @@ hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* sc
     unsigned blocks = grid_for(n, 256);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(dt_reduce_kernel, dim3(blocks), dim3(256), 0, s, g, W, cfl, acc);
-    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, s, acc, dt_out, dt_prev, factor);
+    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, s, acc, dt_out, dt_prev, factor,
+                       scale);
     return hipGetLastError();
 }
 

@@ -93,6 +93,13 @@ class GpuMesh:
     def stage(self, s, dt):
         _native.check(self.lib.fx_stage(self.h, int(s), float(dt)))
 
+    INTEGRATORS = {"rk2": 0, "essprk10_4": 1}
+
+    def set_integrator(self, name):
+        """Time integrator of step/step_async: "rk2" (default) or "essprk10_4"
+        (Ketcheson's eSSPRK(10,4), the reference's default; hydro only)."""
+        _native.check(self.lib.fx_mesh_set_integrator(self.h, self.INTEGRATORS[name]))
+
     def step(self, cfl, dt_factor=1.2):
         dt = C.c_double()
         _native.check(self.lib.fx_step(self.h, float(cfl), float(dt_factor), C.byref(dt)))

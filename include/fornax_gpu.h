@@ -72,6 +72,15 @@ int fx_stage(fx_mesh *m, int stage, double dt);
 int fx_step_async(fx_mesh *m, double cfl, double dt_factor);
 /* Same, then synchronises and returns the dt used. */
 int fx_step(fx_mesh *m, double cfl, double dt_factor, double *dt_used);
+/* Time integrator of fx_step / fx_step_async (TIME_INTEGRATION of
+ * src/prj_defs.h:136-176).  FX_RK2 (default): the two stages above.
+ * FX_ESSPRK10_4: Ketcheson's eSSPRK(10,4), the reference's default, as in
+ * prj_timeint_eSSPRK10_4_advance (src/prj_timeint.c:2661-2730) -- CFL dt x 6,
+ * ten forward-Euler substeps of dt/6 and three convex blends of the conserved
+ * state; hydro meshes only.  fx_stage keeps its RK2 meaning. */
+#define FX_RK2 0
+#define FX_ESSPRK10_4 1
+int fx_mesh_set_integrator(fx_mesh *m, int integrator);
 /* Waits for the mesh's stream and reports any recorded device error. */
 int fx_sync(fx_mesh *m);
 /* dt of the last device step (synchronises). */

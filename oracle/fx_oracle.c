@@ -1236,3 +1236,31 @@ double fxo_calc_dt(const fxo_grid *g, const double *W, double cfl)
     }
     return dt_min;
 }
+
+/* ------------------------------------------------------------------------ */
+/* eSSPRK convex blend (src/prj_timeint.c:2382-2417, hydro): per active cell
+ * u = w_s u(W_saved) + (1 - w_s) u(W), W <- prim(u).                       */
+int fxo_blend(const fxo_grid *g, double *W, const double *Wsaved, double saved_weight)
+{
+    size_t N = ncell(g);
+    const double current_weight = 1.0 - saved_weight;
+    size_t c;
+    if (g->mhd) {
+        return fxo_fail(41, "fxo_blend: hydro only");
+    }
+#pragma omp parallel for schedule(static)
+    for (c = 0; c < N; ++c) {
+        double w[6], ws[6], u[6], us[6];
+        int v;
+        for (v = 0; v < 6; ++v) {
+            w[v] = W[v * N + c];
+            ws[v] = Wsaved[v * N + c];
+        }
+        prim2cons(6, w, u);
+        prim2cons(6, ws, us);
+        for (v = 0; v < 6; ++v) u[v] = saved_weight * us[v] + current_weight * u[v];
+        cons2prim(6, u, w);
+        for (v = 0; v < 6; ++v) W[v * N + c] = w[v];
+    }
+    return 0;
+}

@@ -51,6 +51,9 @@ int fxo_stage(const fxo_grid *g, int stage, double dt,
 
 /* Cell-centred B of active cells from face fields (bf2bc). */
 void fxo_bf2bc(const fxo_grid *g, const double *const *bf, double *W);
+/* eSSPRK convex blend of the conserved state (src/prj_timeint.c:2382-2417),
+ * hydro: W <- prim(w_s u(W_saved) + (1 - w_s) u(W)) */
+int fxo_blend(const fxo_grid *g, double *W, const double *Wsaved, double saved_weight);
 
 /* Per-face and per-edge entry points with the reference's argument meaning. */
 double fxo_mc_slope(double qm, double q0, double qp);

@@ -64,6 +64,7 @@ def oracle_lib():
                                   C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
                                   C.POINTER(_dp)]
         lib.fxo_bf2bc.argtypes = [C.POINTER(_Grid), C.POINTER(_dp), _dp]
+        lib.fxo_blend.argtypes = [C.POINTER(_Grid), _dp, _dp, C.c_double]
         lib.fxo_mc_slope.restype = C.c_double
         lib.fxo_mc_slope.argtypes = [C.c_double] * 3
         lib.fxo_hllc.argtypes = [_dp, _dp] + [C.c_double] * 4 + [_dp, _dp] + [C.c_double] * 3
@@ -152,6 +153,40 @@ class OracleSim:
             if self.mhd:
                 self.bf0 = bfo
 
+    def blend(self, Wsaved, saved_weight):
+        """eSSPRK convex blend of slot 0 with a saved state (hydro)."""
+        rc = self.lib.fxo_blend(C.byref(self.g), _ptr(self.W0), _ptr(Wsaved), float(saved_weight))
+        if rc != 0:
+            raise RuntimeError(self.lib.fxo_last_error().decode())
+
+    def step_essprk10_4(self, cfl, dt_factor=1.2):
+        """Ketcheson eSSPRK(10,4) step (src/prj_timeint.c:2661-2730, hydro):
+        ten forward-Euler substeps of dt/6 with three convex blends.  The CFL
+        step is scaled by the SSP coefficient 6 (src/prj_timeint.c:2250-2251)."""
+        dt_new = self.calc_dt(cfl) * 6.0
+        if self.dt > 0.0:
+            lim = dt_factor * self.dt
+            self.dt = dt_new if dt_new <= lim else lim
+        else:
+            self.dt = dt_new
+        dt_sub = self.dt / 6.0
+
+        def substeps():
+            for _ in range(5):
+                self.stage(1, dt_sub)
+                self.W0 = self.W1.copy()
+
+        s1 = self.W0.copy()
+        substeps()
+        s2 = self.W0.copy()
+        self.blend(s1, 3.0 / 5.0)
+        substeps()
+        self.blend(s2, 3.0 / 8.0)
+        self.blend(s1, 1.0 / 25.0)
+        self.W1 = self.W0.copy()
+        self.time += self.dt
+        return self.dt
+
     def step(self, cfl, dt_factor=1.2):
         """src/main.c:510-545: dt from CFL with the growth limit, then RK2."""
         dt_new = self.calc_dt(cfl)
@@ -169,13 +204,13 @@ class OracleSim:
 _ref_libs = {}
 
 
-def ref_available():
-    return all(os.path.exists(os.path.join(REF_DIR, f"libprj_ref_{k}.so"))
-               for k in ("hydro", "mhd"))
+def ref_available(key=None):
+    keys = (key,) if key else ("hydro", "mhd")
+    return all(os.path.exists(os.path.join(REF_DIR, f"libprj_ref_{k}.so")) for k in keys)
 
 
-def ref_lib(mhd):
-    key = "mhd" if mhd else "hydro"
+def ref_lib(mhd, key=None):
+    key = key or ("mhd" if mhd else "hydro")
     if key not in _ref_libs:
         path = os.path.join(REF_DIR, f"libprj_ref_{key}.so")
         if not os.path.exists(path):
@@ -206,10 +241,11 @@ def ref_lib(mhd):
 
 
 class RefSim:
-    """The serial reference build driven like src/main.c (one mesh per process)."""
+    """The serial reference build driven like src/main.c (one mesh per process).
+    key selects the build: "hydro"/"mhd" (RK2) or "hydro_essprk10_4"."""
 
-    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0):
-        self.lib = ref_lib(mhd)
+    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0, key=None):
+        self.lib = ref_lib(mhd, key)
         nb = [int(v) // 16 for v in n]
         xl = np.ascontiguousarray(xlim, dtype=np.float64)
         bcs = (C.c_int * 6)(*bc)

@@ -78,3 +78,20 @@ def test_mhd_emf_edges():
     for a, b in zip(ref.get_emf(), emf):
         assert np.array_equal(a, b)
     ref.close()
+
+
+@pytest.mark.parametrize("bc", [(3,) * 6, (0,) * 6, (3, 3, 0, 1, 1, 0)])
+def test_essprk10_4_matches_reference_build(bc):
+    """eSSPRK(10,4) steps (CFL dt x 6, ten substeps of dt/6, three convex
+    blends) of the oracle against the reference built with
+    TIME_INTEGRATION=eSSPRK10_4: bit-identical states and dt sequence."""
+    if not O.ref_available("hydro_essprk10_4"):
+        pytest.skip("serial reference build (eSSPRK10_4) not present")
+    pr = P.random_smooth(n=(32, 32, 32), seed=71, bc=bc, gamma=1.4)
+    ref = O.RefSim(pr.n, pr.xlim, pr.bc, pr.gamma, key="hydro_essprk10_4")
+    ref.set_state(pr.W)
+    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma)
+    om.set_state(pr.W)
+    for s in range(3):
+        assert ref.step(pr.cfl) == om.step_essprk10_4(pr.cfl), s
+        assert np.array_equal(ref.get_prim(0), om.W0), s
