@@ -221,10 +221,19 @@ const char* fx_version(void)
     return "fornax_gpu gfx950 fp64 (ffp-contract=off)";
 }
 
+// Cell and shell indices are decomposed in 32-bit arithmetic on the device.
+constexpr long long FX_MAX_CELLS = (1LL << 31) - (1LL << 26);
+
+static bool too_many_cells(const int* n)
+{
+    return (long long)n[0] * n[1] * n[2] > FX_MAX_CELLS;
+}
+
 int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
 {
     if (desc == nullptr || out == nullptr) return fail(FX_EINVAL, "null argument");
     *out = nullptr;
+    if (too_many_cells(desc->n)) return fail(FX_EINVAL, "mesh exceeds 2^31 - 2^26 cells");
     for (int d = 0; d < 3; ++d) {
         if (desc->n[d] <= 0 || desc->n[d] % BS != 0)
             return fail(FX_EINVAL, "grid extent must be a positive multiple of 16");
@@ -635,6 +644,7 @@ int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
             if (lo && d.n[a] < 2 * BS) return fail(FX_EINVAL, "a periodic axis needs at least two blocks");
         }
         if (!(d.gamma > 1.0)) return fail(FX_EINVAL, "gamma must exceed 1");
+        if (too_many_cells(d.n)) return fail(FX_EINVAL, "instance exceeds 2^31 - 2^26 cells");
     }
     fx_ensemble* e = new fx_ensemble();
     e->n = n;

@@ -36,7 +36,6 @@ constexpr int PC = PW * PW;         // 400
 constexpr int NQ = 10;              // rho v1 v2 v3 e ye b1 b2 b3 p
 constexpr int NHALO = PC - BS * BS; // 144
 constexpr int NT = 256;
-constexpr int EX = (BS + 1) * BS;   // 272
 
 __device__ __forceinline__ void set_error(int* err, int code, int dir, int x, int y, int z, int stage)
 {

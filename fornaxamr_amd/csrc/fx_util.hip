@@ -36,33 +36,12 @@ __device__ __forceinline__ double bf_clamped(const Geom& g, const double* bf, in
     return bf[dir * g.fvstride + cell_off(g, x, y, z)];
 }
 
-// One cell of the ghost shell, enumerated as id in [0, shell_cell_count(g)).
-__device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restrict__ W,
-                                                const double* __restrict__ bf, int64_t id)
+// Ghost value of shell cell (x, y, z) from its source cell: 6 hydro variables
+// (normal velocities negated across reflecting faces) and, for MHD, the cell
+// field from the (ghost) faces.
+__device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
+                                         const double* __restrict__ bf, int x, int y, int z)
 {
-    const int nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
-    const int64_t nx_part = 4 * (int64_t)nyp * nzp;
-    const int64_t ny_part = 4 * (int64_t)g.nx * nzp;
-    int x, y, z;
-    if (id < nx_part) {
-        int64_t r = id;
-        z = (int)(r % nzp) - NG; r /= nzp;
-        y = (int)(r % nyp) - NG; r /= nyp;
-        int s = (int)r;                     // 0..3
-        x = s < 2 ? s - NG : g.nx + (s - 2);
-    } else if (id < nx_part + ny_part) {
-        int64_t r = id - nx_part;
-        z = (int)(r % nzp) - NG; r /= nzp;
-        int s = (int)(r % 4); r /= 4;
-        y = s < 2 ? s - NG : g.ny + (s - 2);
-        x = (int)r;
-    } else {
-        int64_t r = id - nx_part - ny_part;
-        int s = (int)(r % 4); r /= 4;
-        z = s < 2 ? s - NG : g.nz + (s - 2);
-        y = (int)(r % g.ny); r /= g.ny;
-        x = (int)r;
-    }
     int flip = 0;
     int mx = map_axis(g, 0, g.nx, x, flip);
     int my = map_axis(g, 1, g.ny, y, flip);
@@ -86,6 +65,70 @@ __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restric
         W[6 * vs + dst] = b1 / 1.0;
         W[7 * vs + dst] = b2 / 1.0;
         W[8 * vs + dst] = b3 / 1.0;
+    }
+}
+
+// One item of the ghost shell, id in [0, shell_cell_count(g)): the x-part
+// (planes x = -2, -1, nx, nx+1 incl. their y/z ghosts) and the y-part (rows
+// y = -2, -1, ny, ny+1 of the interior x range incl. z ghosts) one cell per
+// item, z fastest; the z-part one pair per item -- the two ghost cells at one
+// end of an interior (x, y) row, moved as one 16-byte access per variable
+// (both cells and both sources are 16-byte aligned: rows start on 128 bytes).
+// The decomposition uses 32-bit unsigned arithmetic (a shell has < 2^31
+// items): 64-bit division by a run-time divisor is a long software sequence.
+__device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restrict__ W,
+                                                const double* __restrict__ bf, int64_t id64)
+{
+    const unsigned nyp = (unsigned)(g.ny + 2 * NG), nzp = (unsigned)(g.nz + 2 * NG);
+    const unsigned nx_part = 4u * nyp * nzp;
+    const unsigned ny_part = 4u * (unsigned)g.nx * nzp;
+    const unsigned id = (unsigned)id64;
+    if (id < nx_part) {
+        unsigned r = id;
+        const int z = (int)(r % nzp) - NG; r /= nzp;
+        const int y = (int)(r % nyp) - NG; r /= nyp;
+        const int s = (int)r;                     // 0..3
+        fill_one(g, W, bf, s < 2 ? s - NG : g.nx + (s - 2), y, z);
+        return;
+    }
+    if (id < nx_part + ny_part) {
+        unsigned r = id - nx_part;
+        const int z = (int)(r % nzp) - NG; r /= nzp;
+        const int s = (int)(r & 3u); r >>= 2;
+        fill_one(g, W, bf, (int)r, s < 2 ? s - NG : g.ny + (s - 2), z);
+        return;
+    }
+    unsigned r = id - nx_part - ny_part;
+    const int side = (int)(r & 1u); r >>= 1;
+    const int y = (int)(r % (unsigned)g.ny);
+    const int x = (int)(r / (unsigned)g.ny);
+    const int z0 = side ? g.nz : -NG;             // ghost cells z0, z0 + 1
+    if (g.mhd) {
+        fill_one(g, W, bf, x, y, z0);
+        fill_one(g, W, bf, x, y, z0 + 1);
+        return;
+    }
+    int flip = 0, flip1 = 0;       // both cells of the pair lie outside: same flip
+    const int m0 = map_axis(g, 2, g.nz, z0, flip);
+    const int m1 = map_axis(g, 2, g.nz, z0 + 1, flip1);
+    // periodic: sources m0, m0 + 1 in order; mirror: m1, m1 + 1 reversed
+    const bool rev = m1 < m0;
+    const int64_t src = cell_off(g, x, y, rev ? m1 : m0);
+    const int64_t dst = cell_off(g, x, y, z0);
+    const int64_t vs = g.vstride;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+        double2 val = *reinterpret_cast<const double2*>(W + v * vs + src);
+        if (rev) {
+            const double t = val.x;
+            val.x = val.y;
+            val.y = t;
+        }
+        if (v == 3 && (flip & 4)) {
+            val.x = -val.x;
+            val.y = -val.y;
+        }
+        *reinterpret_cast<double2*>(W + v * vs + dst) = val;
     }
 }
 
@@ -156,10 +199,11 @@ __global__ void __launch_bounds__(256) dt_reduce_kernel(Geom g, const double* __
     double best = 1.0e99;
     for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
          id += (int64_t)gridDim.x * blockDim.x) {
-        int z = (int)(id % g.nz);
-        int64_t r = id / g.nz;
-        int y = (int)(r % g.ny);
-        int x = (int)(r / g.ny);
+        const unsigned u = (unsigned)id;     // cells < 2^31
+        int z = (int)(u % (unsigned)g.nz);
+        unsigned r = u / (unsigned)g.nz;
+        int y = (int)(r % (unsigned)g.ny);
+        int x = (int)(r / (unsigned)g.ny);
         double dt_cell = cell_dt(g, W, cell_off(g, x, y, z), cfl);
         if (dt_cell < best) best = dt_cell;
     }
@@ -301,8 +345,9 @@ __global__ void emf_batch_kernel(int n, const double* ef, const double* ec, cons
 
 int64_t shell_cell_count(const Geom& g)
 {
+    // items: x-part and y-part cells, z-part cell pairs
     const int64_t nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
-    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 4 * (int64_t)g.nx * g.ny;
+    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 2 * (int64_t)g.nx * g.ny;
 }
 
 // eSSPRK convex blend of the active cells (src/prj_timeint.c:2382-2417, hydro):
@@ -314,10 +359,11 @@ __global__ void __launch_bounds__(256) blend_kernel(Geom g, double* __restrict__
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
-    const int z = (int)(id % g.nz);
-    const int64_t r = id / g.nz;
-    const int y = (int)(r % g.ny);
-    const int x = (int)(r / g.ny);
+    const unsigned cid = (unsigned)id;        // cells < 2^31
+    const int z = (int)(cid % (unsigned)g.nz);
+    const unsigned r = cid / (unsigned)g.nz;
+    const int y = (int)(r % (unsigned)g.ny);
+    const int x = (int)(r / (unsigned)g.ny);
     const int64_t o = cell_off(g, x, y, z);
     const int64_t vs = g.vstride;
     const double current_weight = 1.0 - saved_weight;
