@@ -32,6 +32,20 @@ class BlockIO(C.Structure):
                 ("Bv1", C.c_void_p * 3), ("Bv2", C.c_void_p * 3)]
 
 
+class CfBlock(C.Structure):
+    """fx_cf_block: geometry of one block of a refined mesh and its device arrays."""
+    _fields_ = [("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("dx", C.c_double * 3),
+                ("local", C.c_int), ("flux", C.c_void_p * 3), ("emf", C.c_void_p * 3),
+                ("edge_fidelity", C.c_void_p * 3)]
+
+
+class CfLink(C.Structure):
+    """fx_cf_link: one neighbour slot (coarser neighbour) of a local block."""
+    _fields_ = [("block", C.c_int), ("neighbor", C.c_int), ("rel_level", C.c_int),
+                ("type", C.c_int), ("neighbor_local", C.c_int), ("recv_start", C.c_int * 3),
+                ("recv_end", C.c_int * 3), ("send_start", C.c_int * 3)]
+
+
 class FxError(RuntimeError):
     pass
 
@@ -88,6 +102,12 @@ def load():
     lib.fx_flux_update_blocks.argtypes = [C.c_int, vp, C.c_int, vp, vp]
     lib.fx_flux_update_block.argtypes = [C.POINTER(BlockIO), C.c_int]
     lib.fx_flux_div_block.argtypes = [C.POINTER(_dp), _dp, C.c_double, C.c_int, _dp]
+    lib.fx_cf_plan_create.argtypes = [C.c_int, C.POINTER(CfBlock), C.c_int, C.POINTER(CfLink),
+                                      C.c_int, C.POINTER(vp)]
+    lib.fx_cf_plan_info.argtypes = [vp, C.POINTER(C.c_int)]
+    lib.fx_cf_plan_destroy.argtypes = [vp]
+    lib.fx_cf_flux_send.argtypes = [vp, vp]
+    lib.fx_cf_emf_send.argtypes = [vp, vp, vp]
     lib.fx_mesh_set_integrator.argtypes = [vp, C.c_int]
     lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
     lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
@@ -114,5 +134,7 @@ EXPORTS = [
     "fx_adj_create", "fx_adj_destroy", "fx_adj_forward", "fx_adj_backward",
     "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_div_block",
     "fx_mesh_set_integrator",
+    "fx_cf_plan_create", "fx_cf_plan_info", "fx_cf_plan_destroy", "fx_cf_flux_send",
+    "fx_cf_emf_send",
     "fx_last_error", "fx_version",
 ]

@@ -1154,4 +1154,122 @@ int fx_flux_div_block(double* const* flux, const double* area, double vol, int n
     return FX_OK;
 }
 
+// ---- coarse-fine corrections ------------------------------------------------
+
+}  // extern "C"
+
+struct fx_cf_plan {
+    int nblocks = 0;
+    int nvar = 0;
+    void* mem = nullptr;            // one device allocation: table, records, block list
+    CfPtrs* tab = nullptr;
+    CfFlux* flux = nullptr;
+    CfEmf* emf = nullptr;
+    int* fid_blocks = nullptr;
+    int nfid = 0;
+    std::vector<int> flux_off, emf_off;
+    std::string flux_err, emf_err;
+};
+
+extern "C" {
+
+int fx_cf_plan_create(int nblocks, const fx_cf_block* blocks, int nlinks, const fx_cf_link* links,
+                      int nvar, fx_cf_plan** out)
+{
+    if (out == nullptr) return fail(FX_EINVAL, "null plan pointer");
+    *out = nullptr;
+    if (nblocks <= 0 || blocks == nullptr) return fail(FX_EINVAL, "empty block table");
+    if (nlinks < 0 || (nlinks > 0 && links == nullptr)) return fail(FX_EINVAL, "bad link array");
+    if (nvar < 1 || nvar > 16) return fail(FX_EINVAL, "nvar must be 1..16");
+    CfHostPlan hp;
+    cf_build_plan(nblocks, blocks, nlinks, links, nvar, hp);
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_tab = align(sizeof(CfPtrs) * (size_t)nblocks);
+    const size_t b_flux = align(sizeof(CfFlux) * hp.flux.size());
+    const size_t b_emf = align(sizeof(CfEmf) * hp.emf.size());
+    const size_t b_fid = align(sizeof(int) * hp.fid_blocks.size());
+    fx_cf_plan* p = new fx_cf_plan();
+    p->nblocks = nblocks;
+    p->nvar = nvar;
+    hipError_t e = hipMalloc(&p->mem, b_tab + b_flux + b_emf + b_fid);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(e == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,
+                    std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    char* base = (char*)p->mem;
+    p->tab = (CfPtrs*)base;
+    p->flux = (CfFlux*)(base + b_tab);
+    p->emf = (CfEmf*)(base + b_tab + b_flux);
+    p->fid_blocks = (int*)(base + b_tab + b_flux + b_emf);
+    std::vector<CfPtrs> tab(nblocks);
+    for (int b = 0; b < nblocks; ++b)
+        for (int d = 0; d < 3; ++d) {
+            tab[b].flux[d] = blocks[b].flux[d];
+            tab[b].emf[d] = blocks[b].emf[d];
+            tab[b].fid[d] = blocks[b].edge_fidelity[d];
+        }
+    struct Copy { void* dst; const void* src; size_t n; };
+    const Copy copies[] = {{p->tab, tab.data(), sizeof(CfPtrs) * tab.size()},
+                           {p->flux, hp.flux.data(), sizeof(CfFlux) * hp.flux.size()},
+                           {p->emf, hp.emf.data(), sizeof(CfEmf) * hp.emf.size()},
+                           {p->fid_blocks, hp.fid_blocks.data(), sizeof(int) * hp.fid_blocks.size()}};
+    for (const Copy& c : copies) {
+        if (c.n == 0) continue;
+        e = hipMemcpy(c.dst, c.src, c.n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(p->mem);
+            delete p;
+            return hip_fail(e, "hipMemcpy");
+        }
+    }
+    p->nfid = (int)hp.fid_blocks.size();
+    p->flux_off = hp.flux_off;
+    p->emf_off = hp.emf_off;
+    p->flux_err = hp.flux_err;
+    p->emf_err = hp.emf_err;
+    *out = p;
+    return FX_OK;
+}
+
+int fx_cf_plan_info(const fx_cf_plan* p, int* info)
+{
+    if (p == nullptr || info == nullptr) return fail(FX_EINVAL, "null argument");
+    info[0] = p->flux_off.empty() ? 0 : p->flux_off.back();
+    info[1] = p->flux_off.empty() ? 0 : (int)p->flux_off.size() - 1;
+    info[2] = p->emf_off.empty() ? 0 : p->emf_off.back();
+    info[3] = p->emf_off.empty() ? 0 : (int)p->emf_off.size() - 1;
+    return FX_OK;
+}
+
+int fx_cf_plan_destroy(fx_cf_plan* p)
+{
+    if (p == nullptr) return FX_OK;
+    const hipError_t e = p->mem ? hipFree(p->mem) : hipSuccess;
+    delete p;
+    if (e != hipSuccess) return hip_fail(e, "hipFree");
+    return FX_OK;
+}
+
+int fx_cf_flux_send(fx_cf_plan* p, void* stream)
+{
+    if (p == nullptr) return fail(FX_EINVAL, "null plan");
+    if (!p->flux_err.empty()) return fail(FX_EINVAL, "flux correction unavailable: " + p->flux_err);
+    for (size_t l = 0; l + 1 < p->flux_off.size(); ++l)
+        FX_HIP(launch_cf_flux(p->tab, p->flux + p->flux_off[l], p->flux_off[l + 1] - p->flux_off[l],
+                              p->nvar, (hipStream_t)stream));
+    return FX_OK;
+}
+
+int fx_cf_emf_send(fx_cf_plan* p, int* status, void* stream)
+{
+    if (p == nullptr) return fail(FX_EINVAL, "null plan");
+    if (!p->emf_err.empty()) return fail(FX_EINVAL, "EMF restriction unavailable: " + p->emf_err);
+    FX_HIP(launch_cf_fid_init(p->tab, p->fid_blocks, p->nfid, (hipStream_t)stream));
+    for (size_t l = 0; l + 1 < p->emf_off.size(); ++l)
+        FX_HIP(launch_cf_emf(p->tab, p->emf + p->emf_off[l], p->emf_off[l + 1] - p->emf_off[l],
+                             status, (hipStream_t)stream));
+    return FX_OK;
+}
+
 }  // extern "C"

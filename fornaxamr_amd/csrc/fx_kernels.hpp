@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+#include <vector>
+
 #include "../../include/fornax_gpu.h"
 #include "fx_layout.hpp"
 
@@ -126,5 +129,33 @@ hipError_t launch_block_flux(const fx_block_io* blocks_dev, int n, int mhd, int*
 hipError_t launch_block_flux_div(const double* f0, const double* f1, const double* f2,
                                  const double* area, double vol, int nvar, double* out,
                                  hipStream_t s);
+
+// ---- coarse-fine corrections of refined meshes (fx_amr.hip) ---------------
+struct CfPtrs {             // device arrays of one block of the plan's table
+    double* flux[3];
+    double* emf[3];
+    int* fid[3];
+};
+struct CfFlux {             // one coarse face cell: dst = 0.25 (s0 + s1 + s2 + s3)
+    int dst, src, axis, dst_off;
+    int s[4];               // BIDX(nvar, 0, ...) of the four fine faces
+};
+struct CfEmf {              // one coarse edge: dst = ((0 + a) + b) / 2
+    int dst, src, dir, dst_idx;
+    int a, b, store, pad;   // store 0: overwritten later, finiteness check only
+};
+struct CfHostPlan {
+    std::vector<CfFlux> flux;
+    std::vector<int> flux_off;      // level l: records [off[l], off[l+1])
+    std::vector<CfEmf> emf;
+    std::vector<int> emf_off;
+    std::vector<int> fid_blocks;    // local blocks whose edge fidelity is reset
+    std::string flux_err, emf_err;  // why a part is unavailable ("" if usable)
+};
+std::string cf_build_plan(int nblocks, const fx_cf_block* blocks, int nlinks,
+                          const fx_cf_link* links, int nvar, CfHostPlan& p);
+hipError_t launch_cf_flux(const CfPtrs* tab, const CfFlux* recs, int n, int nvar, hipStream_t s);
+hipError_t launch_cf_fid_init(const CfPtrs* tab, const int* blocks, int n, hipStream_t s);
+hipError_t launch_cf_emf(const CfPtrs* tab, const CfEmf* recs, int n, int* status, hipStream_t s);
 
 }  // namespace fx

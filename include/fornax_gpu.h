@@ -159,6 +159,54 @@ int fx_flux_update_block(const fx_block_io *block, int mhd);
 int fx_flux_div_block(double *const *flux, const double *area, double vol, int nvar,
     double *fluxdiv);
 
+/* ---- coarse-fine corrections of a refined mesh (per-block layouts) ------ *
+ * The local phase of prj_riemann_flux_send (src/prj_riemann.h:25,
+ * src/prj_riemann.c:1701-1768) and prj_mhd_emf_send (src/prj_mhd.h,
+ * src/prj_mhd.c:779-1029) on device-resident block arrays.
+ *
+ * The caller describes the mesh once per topology: a table of blocks (index
+ * = the reference's block id; geometry of every block, arrays of the local
+ * ones) and the neighbour slots of its local blocks whose neighbour is
+ * coarser, in the reference's loop order (blocks by id, slots 0..55).
+ * fx_cf_plan_create turns them into device correction lists; each stage
+ * then calls fx_cf_flux_send after the flux update and fx_cf_emf_send after
+ * the edge EMFs, both stream-ordered, neither synchronising.  The results,
+ * the order of overlapping writes included, are the reference's. */
+typedef struct fx_cf_block {
+    double xmin[3], xmax[3], dx[3];  /* prj_block xmin, xmax, dx */
+    int local;                       /* id >= 0, active, owned by this rank */
+    double *flux[3];                 /* device flux[axis], nvar x 8000 (VIDX), or NULL */
+    double *emf[3];                  /* device emf[dir], 8820 (EDGE_IDX), or NULL */
+    int *edge_fidelity[3];           /* device edge_fidelity[dir], or NULL (not kept) */
+} fx_cf_block;
+
+typedef struct fx_cf_link {          /* slot n of a local block (src/prj_types.h:77-105) */
+    int block;                       /* the block's table index */
+    int neighbor;                    /* slot->id */
+    int rel_level;                   /* slot->rel_level; >= 0 is skipped */
+    int type;                        /* slot->type: 1 face, 2 edge, 3 corner */
+    int neighbor_local;              /* slot->rank is this rank */
+    int recv_start[3];               /* slot->recv_loc_start */
+    int recv_end[3];                 /* slot->recv_loc_end */
+    int send_start[3];               /* slot->send_loc_start */
+} fx_cf_link;
+
+typedef struct fx_cf_plan fx_cf_plan;
+/* nvar: flux components (NVAR_CONS: 6 hydro, 9 MHD).  A part whose arrays
+ * are missing stays unavailable: its send call then fails with FX_EINVAL. */
+int fx_cf_plan_create(int nblocks, const fx_cf_block *blocks, int nlinks,
+    const fx_cf_link *links, int nvar, fx_cf_plan **out);
+/* info[4] = flux records, flux launch levels, EMF records, EMF launch levels */
+int fx_cf_plan_info(const fx_cf_plan *p, int *info);
+int fx_cf_plan_destroy(fx_cf_plan *p);
+/* prj_riemann_flux_send, local phase: coarse face fluxes from the fine faces */
+int fx_cf_flux_send(fx_cf_plan *p, void *stream);
+/* prj_mhd_emf_send: edge fidelity reset, coarse edge EMFs from the fine
+ * edges.  status (device int[4], zeroed by the caller, or NULL) receives the
+ * first non-finite restricted value {1, block, dir, edge index}, where the
+ * reference aborts with "prj_mhd_emf_send: invalid emf write". */
+int fx_cf_emf_send(fx_cf_plan *p, int *status, void *stream);
+
 /* ---- ragged ensembles of independent hydro meshes (batched stage) ------- *
  * No counterpart in the reference, which runs one problem per process: an
  * ensemble advances n independent periodic/outflow/reflect hydro meshes of

@@ -78,6 +78,9 @@ def oracle_lib():
                                     _dp]
         lib.fxa_tlm.argtypes = [_ip, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                 _dp, _dp]
+        vpp = C.POINTER(C.c_void_p)
+        lib.fxo_cf_flux_send.argtypes = [C.c_int, _dp, _ip, _ip, C.c_int, vpp, C.c_int]
+        lib.fxo_cf_emf_send.argtypes = [C.c_int, _dp, _ip, _ip, vpp, vpp, C.c_int]
         _oracle_lib = lib
     return _oracle_lib
 
@@ -448,3 +451,167 @@ def ref_flux_div_raw(mhd, flux):
     vol = np.empty(1)
     lib.refh_block_geometry(_ptr(area), _ptr(vol))
     return out, area, float(vol[0])
+
+
+# ---- refined meshes: coarse-fine flux correction and EMF restriction ---------
+RB_EDGES = 21 * 21 * 20      # PRJ_BLOCK_NEDGES at 16^3 blocks, 2 ghost layers
+
+
+class AmrMesh:
+    """Topology of a refined mesh as the reference holds it: ``geo`` (nb, 9) =
+    xmin, xmax, dx; ``info`` (nb, 4) = id, active, level, rank; ``slots``
+    (nb, 56, 16) = id, rank, rel_level, type, send_loc_start[3],
+    send_loc_end[3], recv_loc_start[3], recv_loc_end[3] (src/prj_types.h:77-105)."""
+
+    def __init__(self, geo, info, slots):
+        self.geo = np.ascontiguousarray(geo, dtype=np.float64)
+        self.info = np.ascontiguousarray(info, dtype=np.int32)
+        self.slots = np.ascontiguousarray(slots, dtype=np.int32)
+        self.nblocks = len(self.geo)
+
+    @classmethod
+    def from_reference(cls, root, xlim, bc, max_level, ball, max_blocks=4000, mhd=True):
+        """Build with the reference's own AMR code (refh_amr_init); the
+        reference build keeps the mesh for the ref_cf_* calls."""
+        lib = ref_amr_lib(mhd)
+        rc = lib.refh_amr_init(*[int(r) for r in root], _ptr(np.asarray(xlim, float)),
+                               (C.c_int * 6)(*[int(b) for b in bc]), int(max_level),
+                               _ptr(np.asarray(ball, float)), int(max_blocks))
+        if rc:
+            raise RuntimeError(f"refh_amr_init failed ({rc})")
+        nb = lib.refh_amr_nblocks()
+        geo = np.zeros((nb, 9))
+        info = np.zeros((nb, 4), np.int32)
+        slots = np.zeros((nb, 56, 16), np.int32)
+        for b in range(nb):
+            lib.refh_amr_block(b, _ptr(geo[b]), info[b].ctypes.data_as(_ip))
+            lib.refh_amr_slots(b, slots[b].ctypes.data_as(_ip))
+        return cls(geo, info, slots)
+
+    def save(self, path):
+        np.savez_compressed(path, geo=self.geo, info=self.info, slots=self.slots)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path)
+        return cls(z["geo"], z["info"], z["slots"])
+
+    def local(self, rank=0):
+        return (self.info[:, 0] >= 0) & (self.info[:, 1] == 1) & (self.info[:, 3] == rank)
+
+
+def _ptr_table(arrays):
+    """void*[nb*3] of per-block arrays (None -> NULL)."""
+    t = (C.c_void_p * len(arrays))()
+    for i, a in enumerate(arrays):
+        t[i] = None if a is None else a.ctypes.data
+    return t
+
+
+def cf_flux_send(mesh, flux, nvar, rank=0):
+    """Oracle prj_riemann_flux_send (local phase) on flux[b][axis] arrays,
+    in place (oracle/fxcf_oracle.c)."""
+    arrs = [flux[b][a] if flux[b] is not None else None for b in range(mesh.nblocks) for a in range(3)]
+    oracle_lib().fxo_cf_flux_send(mesh.nblocks, _ptr(mesh.geo), mesh.info.ctypes.data_as(_ip),
+                                  mesh.slots.ctypes.data_as(_ip), int(nvar), _ptr_table(arrs),
+                                  int(rank))
+
+
+def cf_emf_send(mesh, emf, fid, rank=0):
+    """Oracle prj_mhd_emf_send on emf[b][dir] / fid[b][dir], in place.
+    Returns 1 where the reference would abort on a non-finite value."""
+    e = [emf[b][d] if emf[b] is not None else None for b in range(mesh.nblocks) for d in range(3)]
+    f = [fid[b][d] if fid is not None and fid[b] is not None else None
+         for b in range(mesh.nblocks) for d in range(3)]
+    return oracle_lib().fxo_cf_emf_send(mesh.nblocks, _ptr(mesh.geo), mesh.info.ctypes.data_as(_ip),
+                                        mesh.slots.ctypes.data_as(_ip), _ptr_table(e),
+                                        _ptr_table(f) if fid is not None else None, int(rank))
+
+
+def ref_amr_lib(mhd):
+    lib = ref_lib(mhd)
+    if not getattr(lib, "_amr_ready", False):
+        lib.refh_amr_init.argtypes = [C.c_int] * 3 + [_dp, _ip, C.c_int, _dp, C.c_int]
+        lib.refh_amr_block.argtypes = [C.c_int, _dp, _ip]
+        lib.refh_amr_slots.argtypes = [C.c_int, _ip]
+        lib.refh_amr_flux.argtypes = [C.c_int, C.c_int, _dp, C.c_int]
+        if mhd:
+            lib.refh_amr_emf.argtypes = [C.c_int, C.c_int, _dp, C.c_int]
+            lib.refh_amr_fidelity.argtypes = [C.c_int, C.c_int, _ip]
+        lib._amr_ready = True
+    return lib
+
+
+def ref_cf_flux_send(mhd, mesh, flux):
+    """prj_riemann_flux_send of the reference build on the mesh last built by
+    AmrMesh.from_reference(mhd=mhd): flux[b] (3 arrays) of every block with
+    storage is copied in, the reference runs, and the arrays are read back."""
+    lib = ref_amr_lib(mhd)
+    out = []
+    for b in range(mesh.nblocks):
+        if flux[b] is None:
+            out.append(None)
+            continue
+        for a in range(3):
+            if lib.refh_amr_flux(b, a, _ptr(np.ascontiguousarray(flux[b][a])), 1):
+                raise RuntimeError(f"block {b} has no flux storage")
+        out.append(None)
+    lib.refh_amr_flux_send()
+    for b in range(mesh.nblocks):
+        if flux[b] is None:
+            continue
+        res = [np.empty_like(flux[b][a]) for a in range(3)]
+        for a in range(3):
+            lib.refh_amr_flux(b, a, _ptr(res[a]), 0)
+        out[b] = res
+    return out
+
+
+def ref_cf_emf_send(mesh, emf):
+    """prj_mhd_emf_send of the MHD reference build; returns (emf, fidelity)."""
+    lib = ref_amr_lib(True)
+    for b in range(mesh.nblocks):
+        if emf[b] is None:
+            continue
+        for d in range(3):
+            if lib.refh_amr_emf(b, d, _ptr(np.ascontiguousarray(emf[b][d])), 1):
+                raise RuntimeError(f"block {b} has no emf storage")
+    lib.refh_amr_emf_send()
+    e_out, f_out = [], []
+    for b in range(mesh.nblocks):
+        if emf[b] is None:
+            e_out.append(None)
+            f_out.append(None)
+            continue
+        e = [np.empty(RB_EDGES) for _ in range(3)]
+        f = [np.empty(RB_EDGES, np.int32) for _ in range(3)]
+        for d in range(3):
+            lib.refh_amr_emf(b, d, _ptr(e[d]), 0)
+            lib.refh_amr_fidelity(b, d, f[d].ctypes.data_as(_ip))
+        e_out.append(e)
+        f_out.append(f)
+    return e_out, f_out
+
+
+def cf_inputs(mesh, nvar, seed):
+    """Seeded per-block flux arrays (3 per block, nvar x 8000 each) for every
+    block of the mesh."""
+    rng = np.random.default_rng(seed)
+    return [[rng.standard_normal(nvar * RB_CELLS) for _ in range(3)] for _ in range(mesh.nblocks)]
+
+
+def cf_emf_inputs(mesh, seed):
+    rng = np.random.default_rng(seed)
+    return [[rng.standard_normal(RB_EDGES) for _ in range(3)] for _ in range(mesh.nblocks)]
+
+
+def cf_digest(arrays, blocks=None):
+    """SHA-256 over the per-block arrays in block order (axis / dir fastest)."""
+    import hashlib
+    h = hashlib.sha256()
+    for b, arrs in enumerate(arrays):
+        if arrs is None or (blocks is not None and not blocks[b]):
+            continue
+        for a in arrs:
+            h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()

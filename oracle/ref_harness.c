@@ -502,3 +502,176 @@ void refh_block_geometry(double *area, double *vol)
     area[2] = b->area[2];
     *vol = b->vol;
 }
+
+/* ---- refined meshes: coarse-fine flux correction and EMF restriction ----
+ *
+ * refh_amr_init builds a two-or-more-level mesh with the reference's own AMR
+ * code: root blocks from prj_mesh_init, then, level by level, blocks whose box
+ * meets the ball (cx, cy, cz, r) are marked, the 2:1 balance is enforced and
+ * the marked blocks refined, and the neighbour slots rebuilt -- the loop of
+ * problems/prj_problem_sedov.c:134-176.  Block data of every new block is
+ * allocated by prj_amr_refine_block itself (single rank). */
+static int refh_box_meets_ball(const prj_block *b, const double *ball)
+{
+    double d2 = 0.0;
+    int a;
+    for (a = 0; a < 3; ++a) {
+        double c = ball[a];
+        double q = c < b->xmin[a] ? b->xmin[a] - c : (c > b->xmax[a] ? c - b->xmax[a] : 0.0);
+        d2 += q * q;
+    }
+    return d2 < ball[3] * ball[3];
+}
+
+int refh_amr_init(int nbx, int nby, int nbz, const double *xlim, const int *bc,
+    int max_level, const double *ball, int max_blocks)
+{
+    int axis;
+    int level;
+
+    refh_free();
+    memset(&g_sim, 0, sizeof(g_sim));
+    memset(&g_mpi, 0, sizeof(g_mpi));
+    g_mpi.rank = 0;
+    g_mpi.totrank = 1;
+    g_nb[0] = nbx;
+    g_nb[1] = nby;
+    g_nb[2] = nbz;
+    g_sim.coord.x1min = xlim[0];
+    g_sim.coord.x1max = xlim[1];
+    g_sim.coord.x2min = xlim[2];
+    g_sim.coord.x2max = xlim[3];
+    g_sim.coord.x3min = xlim[4];
+    g_sim.coord.x3max = xlim[5];
+    g_sim.bc.bc_x1_inner = bc[0];
+    g_sim.bc.bc_x1_outer = bc[1];
+    g_sim.bc.bc_x2_inner = bc[2];
+    g_sim.bc.bc_x2_outer = bc[3];
+    g_sim.bc.bc_x3_inner = bc[4];
+    g_sim.bc.bc_x3_outer = bc[5];
+    for (axis = 0; axis < 3; ++axis) {
+        int inner_p = bc[2 * axis] == PRJ_BC_PERIODIC;
+        int outer_p = bc[2 * axis + 1] == PRJ_BC_PERIODIC;
+        if (inner_p != outer_p) return 1;
+        g_sim.mesh.periodic[axis] = inner_p;
+    }
+    g_sim.mesh.max_blocks = max_blocks;
+    if (prj_mesh_init(&g_sim.mesh, nbx, nby, nbz, max_level, &g_sim.coord, 0) != 0) {
+        return 2;
+    }
+    prj_mpi_decompose(&g_sim.mesh, &g_mpi);
+    prj_amr_init_neighbors(&g_sim.mesh);
+    for (level = 0; level < max_level; ++level) {
+        int bidx;
+        int any = 0;
+        for (bidx = 0; bidx < g_sim.mesh.nblocks; ++bidx) {
+            prj_block *b = &g_sim.mesh.blocks[bidx];
+            if (b->id < 0 || b->active != 1 || b->level != level) continue;
+            b->refine_flag = refh_box_meets_ball(b, ball) ? 1 : 0;
+            any |= b->refine_flag;
+        }
+        if (!any) break;
+        prj_amr_enforce_two_to_one(&g_sim.mesh, &g_mpi);
+        if (prj_amr_refine_marked_blocks(&g_sim.mesh, &g_mpi) == 0) break;
+        prj_amr_init_neighbors(&g_sim.mesh);
+    }
+    prj_mesh_update_max_active_level(&g_sim.mesh);
+    g_ready = 1;
+    return 0;
+}
+
+int refh_amr_nblocks(void)
+{
+    return g_ready ? g_sim.mesh.nblocks : -1;
+}
+
+/* geo[9] = xmin[3], xmax[3], dx[3]; info[4] = id, active, level, rank */
+int refh_amr_block(int id, double *geo, int *info)
+{
+    const prj_block *b;
+    int d;
+    if (!g_ready || id < 0 || id >= g_sim.mesh.nblocks) return 1;
+    b = &g_sim.mesh.blocks[id];
+    for (d = 0; d < 3; ++d) {
+        geo[d] = b->xmin[d];
+        geo[3 + d] = b->xmax[d];
+        geo[6 + d] = b->dx[d];
+    }
+    info[0] = b->id;
+    info[1] = b->active;
+    info[2] = b->level;
+    info[3] = b->rank;
+    return 0;
+}
+
+/* out[56][16]: id, rank, rel_level, type, send_loc_start[3], send_loc_end[3],
+ * recv_loc_start[3], recv_loc_end[3] of every neighbour slot (src/prj_types.h:77-105). */
+int refh_amr_slots(int id, int *out)
+{
+    const prj_block *b;
+    int n, d;
+    if (!g_ready || id < 0 || id >= g_sim.mesh.nblocks) return 1;
+    b = &g_sim.mesh.blocks[id];
+    for (n = 0; n < 56; ++n) {
+        const prj_neighbor *s = &b->slot[n];
+        int *o = out + 16 * n;
+        o[0] = s->id;
+        o[1] = s->rank;
+        o[2] = s->rel_level;
+        o[3] = s->type;
+        for (d = 0; d < 3; ++d) {
+            o[4 + d] = s->send_loc_start[d];
+            o[7 + d] = s->send_loc_end[d];
+            o[10 + d] = s->recv_loc_start[d];
+            o[13 + d] = s->recv_loc_end[d];
+        }
+    }
+    return 0;
+}
+
+/* Copy block->flux[axis] (NVAR_CONS x NCELLS, VIDX) in (set = 1) or out. */
+int refh_amr_flux(int id, int axis, double *buf, int set)
+{
+    prj_block *b;
+    size_t n = (size_t)PRJ_NVAR_CONS * PRJ_BLOCK_NCELLS;
+    if (!g_ready || id < 0 || id >= g_sim.mesh.nblocks || axis < 0 || axis > 2) return 1;
+    b = &g_sim.mesh.blocks[id];
+    if (b->flux[axis] == 0) return 2;
+    if (set) memcpy(b->flux[axis], buf, n * sizeof(double));
+    else memcpy(buf, b->flux[axis], n * sizeof(double));
+    return 0;
+}
+
+void refh_amr_flux_send(void)
+{
+    prj_riemann_flux_send(&g_sim.mesh, &g_mpi);
+}
+
+#if PRJ_MHD
+/* Copy block->emf[dir] (NEDGES, EDGE_IDX) in (set = 1) or out. */
+int refh_amr_emf(int id, int dir, double *buf, int set)
+{
+    prj_block *b;
+    if (!g_ready || id < 0 || id >= g_sim.mesh.nblocks || dir < 0 || dir > 2) return 1;
+    b = &g_sim.mesh.blocks[id];
+    if (b->emf[dir] == 0) return 2;
+    if (set) memcpy(b->emf[dir], buf, (size_t)PRJ_BLOCK_NEDGES * sizeof(double));
+    else memcpy(buf, b->emf[dir], (size_t)PRJ_BLOCK_NEDGES * sizeof(double));
+    return 0;
+}
+
+int refh_amr_fidelity(int id, int dir, int *out)
+{
+    const prj_block *b;
+    if (!g_ready || id < 0 || id >= g_sim.mesh.nblocks || dir < 0 || dir > 2) return 1;
+    b = &g_sim.mesh.blocks[id];
+    if (b->edge_fidelity[dir] == 0) return 2;
+    memcpy(out, b->edge_fidelity[dir], (size_t)PRJ_BLOCK_NEDGES * sizeof(int));
+    return 0;
+}
+
+void refh_amr_emf_send(void)
+{
+    prj_mhd_emf_send(&g_sim.mesh, &g_mpi);
+}
+#endif

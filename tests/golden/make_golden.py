@@ -7,9 +7,13 @@ prj_riemann_hllc, prj_riemann_hlld and prj_mhd_emf_upwind, and, for whole
 RK2 trajectories, the SHA-256 of the input state and of the state after every
 step plus the dt sequence and a few sampled values; for the per-block entry
 points, the SHA-256 of the reference's prj_flux_update / prj_flux_div outputs
-on seeded raw blocks.
+on seeded raw blocks; for the coarse-fine corrections of refined meshes,
+the mesh topologies the reference's own AMR code builds (cf_mesh_*.npz:
+block geometry, levels and neighbour slots) and the SHA-256 of the
+reference's prj_riemann_flux_send / prj_mhd_emf_send outputs on seeded
+block arrays (cf_corrections.json).
 
-    python tests/golden/make_golden.py [--only-block]
+    python tests/golden/make_golden.py [--only-block | --only-cf]
 """
 from __future__ import annotations
 
@@ -181,7 +185,50 @@ def block_flux():
     return recs
 
 
+CF_MESHES = {
+    # two levels, outflow
+    "two_level": dict(root=(2, 2, 2), xlim=(-1, 1, -1, 1, -1, 1), bc=(0,) * 6, max_level=2,
+                      ball=(0.3, 0.2, -0.4, 0.2)),
+    # same refinement, periodic: slots across the seam see the far block
+    "two_level_periodic": dict(root=(2, 2, 2), xlim=(-1, 1, -1, 1, -1, 1), bc=(3,) * 6,
+                               max_level=2, ball=(0.3, 0.2, -0.4, 0.2)),
+    # three active levels on a 2x1x1 root (blocks that both receive and send)
+    "three_level": dict(root=(2, 1, 1), xlim=(-2, 2, -1, 1, -1, 1), bc=(0,) * 6, max_level=3,
+                        ball=(0.55, 0.3, -0.45, 0.03)),
+}
+CF_SEEDS = {"flux6": 101, "flux9": 102, "emf": 103}
+
+
+def cf_fixtures():
+    recs = {}
+    for name, cfg in CF_MESHES.items():
+        rec = dict(cfg)
+        mesh = O.AmrMesh.from_reference(mhd=False, **cfg)
+        mesh.save(os.path.join(HERE, f"cf_mesh_{name}.npz"))
+        flux = O.cf_inputs(mesh, 6, CF_SEEDS["flux6"])
+        rec["flux6_in"] = O.cf_digest(flux)
+        rec["flux6_out"] = O.cf_digest(O.ref_cf_flux_send(False, mesh, flux))
+        mesh_m = O.AmrMesh.from_reference(mhd=True, **cfg)
+        assert np.array_equal(mesh_m.slots, mesh.slots) and np.array_equal(mesh_m.geo, mesh.geo)
+        flux = O.cf_inputs(mesh, 9, CF_SEEDS["flux9"])
+        rec["flux9_in"] = O.cf_digest(flux)
+        rec["flux9_out"] = O.cf_digest(O.ref_cf_flux_send(True, mesh_m, flux))
+        emf = O.cf_emf_inputs(mesh, CF_SEEDS["emf"])
+        rec["emf_in"] = O.cf_digest(emf)
+        e, f = O.ref_cf_emf_send(mesh_m, emf)
+        loc = mesh.local()
+        rec["emf_out"] = O.cf_digest(e)
+        rec["fidelity_local_out"] = O.cf_digest(f, loc)
+        rec["seeds"] = CF_SEEDS
+        recs[name] = rec
+    with open(os.path.join(HERE, "cf_corrections.json"), "w") as fh:
+        json.dump(recs, fh, indent=1)
+
+
 def main():
+    if "--only-cf" in sys.argv:
+        cf_fixtures()
+        return
     if "--only-block" in sys.argv:
         with open(os.path.join(HERE, "block_flux.json"), "w") as f:
             json.dump(block_flux(), f, indent=1)
@@ -199,6 +246,7 @@ def main():
         json.dump(recs, f, indent=1)
     with open(os.path.join(HERE, "block_flux.json"), "w") as f:
         json.dump(block_flux(), f, indent=1)
+    cf_fixtures()
     print("wrote", sorted(os.listdir(HERE)))
 
 
