@@ -15,6 +15,10 @@ inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / b
 // src/prj_mesh.c:1597-1645) or the mirror copy of the physical boundary
 // (src/prj_boundary.c:413-487: src = -a-1 / 2N-1-a), reflect negating the
 // normal velocity.
+// cells written at each end of a row by the hydro z-part of the ghost fill
+constexpr int ZEND = 8;
+static_assert(ZEND <= PADZ && ZEND <= BS, "z-end segment must fit the padding and a block");
+
 __device__ __forceinline__ int map_axis(const Geom& g, int axis, int n, int c, int& flip)
 {
     if (c >= 0 && c < n) return c;
@@ -71,9 +75,14 @@ __device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
 // One item of the ghost shell, id in [0, shell_cell_count(g)): the x-part
 // (planes x = -2, -1, nx, nx+1 incl. their y/z ghosts) and the y-part (rows
 // y = -2, -1, ny, ny+1 of the interior x range incl. z ghosts) one cell per
-// item, z fastest; the z-part one pair per item -- the two ghost cells at one
-// end of an interior (x, y) row, moved as one 16-byte access per variable
-// (both cells and both sources are 16-byte aligned: rows start on 128 bytes).
+// item, z fastest; the z-part one pair of cells per item, as one 16-byte
+// access per variable (cells and sources are 16-byte aligned: rows start on
+// 128 bytes).  Hydro: the z-part covers the 8 cells z = -8..-1 and
+// nz..nz+7 at each end of an interior (x, y) row -- 4 items per end, adjacent
+// lanes -- so each end is one whole aligned 64-byte segment per variable
+// instead of a partial one; cells beyond the 2 ghost layers lie in the z
+// padding, which nothing reads.  MHD (cell B from the faces): z = -2, -1 and
+// nz, nz+1 only, one item per end.
 // The decomposition uses 32-bit unsigned arithmetic (a shell has < 2^31
 // items): 64-bit division by a run-time divisor is a long software sequence.
 __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restrict__ W,
@@ -99,15 +108,20 @@ __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restric
         return;
     }
     unsigned r = id - nx_part - ny_part;
-    const int side = (int)(r & 1u); r >>= 1;
-    const int y = (int)(r % (unsigned)g.ny);
-    const int x = (int)(r / (unsigned)g.ny);
-    const int z0 = side ? g.nz : -NG;             // ghost cells z0, z0 + 1
     if (g.mhd) {
+        const int side = (int)(r & 1u); r >>= 1;
+        const int y = (int)(r % (unsigned)g.ny);
+        const int x = (int)(r / (unsigned)g.ny);
+        const int z0 = side ? g.nz : -NG;         // ghost cells z0, z0 + 1
         fill_one(g, W, bf, x, y, z0);
         fill_one(g, W, bf, x, y, z0 + 1);
         return;
     }
+    const int q = (int)(r & 3u);                  // pair within the 8-cell end
+    const int side = (int)((r >> 2) & 1u); r >>= 3;
+    const int y = (int)(r % (unsigned)g.ny);
+    const int x = (int)(r / (unsigned)g.ny);
+    const int z0 = (side ? g.nz : -ZEND) + 2 * q; // cells z0, z0 + 1
     int flip = 0, flip1 = 0;       // both cells of the pair lie outside: same flip
     const int m0 = map_axis(g, 2, g.nz, z0, flip);
     const int m1 = map_axis(g, 2, g.nz, z0 + 1, flip1);
@@ -345,9 +359,10 @@ __global__ void emf_batch_kernel(int n, const double* ef, const double* ec, cons
 
 int64_t shell_cell_count(const Geom& g)
 {
-    // items: x-part and y-part cells, z-part cell pairs
+    // items: x-part and y-part cells, z-part cell pairs (hydro: 4 per row end)
     const int64_t nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
-    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 2 * (int64_t)g.nx * g.ny;
+    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp +
+           (g.mhd ? 2 : 2 * (ZEND / 2)) * (int64_t)g.nx * g.ny;
 }
 
 // eSSPRK convex blend of the active cells (src/prj_timeint.c:2382-2417, hydro):
