@@ -153,7 +153,8 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
     const Geom& g = m->g;
     double* Win = stage == 1 ? m->W[0] : m->W[1];
     if (!g.mhd) {
-        FX_HIP(launch_boundary_fill(g, Win, nullptr, m->stream));
+        // the hydro stage reads cells outside the domain at their sources:
+        // no ghost fill
         HydroStageArgs a;
         a.g = g;
         a.Win = Win;
@@ -545,10 +546,7 @@ struct fx_ensemble {
     EnsInst* d_inst = nullptr;
     EnsBlock* d_blocks[2] = {nullptr, nullptr};
     int* d_queues = nullptr;
-    int2* d_fill = nullptr;              // ghost-fill chunks (instance, first shell id)
-    int nfill = 0;
     double* dev = nullptr;               // 4 doubles per instance
-    int64_t total_shell = 0;
     int nblocks = 0;
     hipStream_t stream = nullptr;        // owned: graph capture needs a non-null stream
     // schedule: mode 0 = one launch set per instance, 1 = batched
@@ -582,7 +580,6 @@ int ens_enqueue_step(fx_ensemble* e, double cfl, double factor)
             FX_HIP(launch_calc_dt(I.g, I.W[0], cfl, I.dev + 2, I.dev + 0, I.dev + 1, factor, s));
             for (int st = 1; st <= 2; ++st) {
                 double* Win = st == 1 ? I.W[0] : I.W[1];
-                FX_HIP(launch_boundary_fill(I.g, Win, nullptr, s));
                 HydroStageArgs a;
                 a.g = I.g;
                 a.Win = Win;
@@ -597,11 +594,9 @@ int ens_enqueue_step(fx_ensemble* e, double cfl, double factor)
     }
     const EnsBlock* tbl = e->d_blocks[e->table];
     FX_HIP(launch_ens_dt(e->d_inst, e->n, tbl, e->nblocks, cfl, factor, s));
-    for (int st = 1; st <= 2; ++st) {
-        FX_HIP(launch_ens_fill(e->d_inst, e->d_fill, e->nfill, st == 1 ? 0 : 1, s));
+    for (int st = 1; st <= 2; ++st)
         FX_HIP(launch_ens_stage(e->d_inst, tbl, e->nblocks, st, e->persistent_wgs, e->xcd, e->xcd,
                                 e->d_queues, s));
-    }
     return FX_OK;
 }
 
@@ -622,7 +617,6 @@ int fx_ensemble_destroy(fx_ensemble* e)
     (void)hipFree(e->d_blocks[0]);
     (void)hipFree(e->d_blocks[1]);
     (void)hipFree(e->d_queues);
-    (void)hipFree(e->d_fill);
     (void)hipFree(e->dev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -651,7 +645,6 @@ int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
     hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipMalloc(&e->dev, sizeof(double) * 4 * n);
     if (err == hipSuccess) err = hipMemset(e->dev, 0, sizeof(double) * 4 * n);
-    int64_t shell = 0;
     for (int i = 0; i < n && err == hipSuccess; ++i) {
         const fx_mesh_desc& d = descs[i];
         Geom g = make_geom(d.n[0], d.n[1], d.n[2], d.xlim, d.bc, d.gamma, 0, 0.0);
@@ -663,16 +656,12 @@ int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
         if (err == hipSuccess) err = hipMalloc(&I.W[1], bytes);
         if (err == hipSuccess) err = hipMemset(I.W[1], 0, bytes);
         I.dev = e->dev + 4 * i;
-        I.shell_begin = shell;
-        I.shell_count = shell_cell_count(g);
-        shell += I.shell_count;
         e->geoms.push_back(g);
         e->host_inst.push_back(I);
         for (int bx = 0; bx < g.nbx; ++bx)
             for (int by = 0; by < g.nby; ++by)
                 for (int bz = 0; bz < g.nbz; ++bz) e->order[0].push_back({i, bx, by, bz});
     }
-    e->total_shell = shell;
     e->nblocks = (int)e->order[0].size();
     // interleaved table: instances dealt round robin block by block, so that
     // every part of the table mixes all instances
@@ -704,16 +693,6 @@ int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
         if (err == hipSuccess)
             err = hipMemcpy(e->d_blocks[t], e->order[t].data(), sizeof(EnsBlock) * e->nblocks,
                             hipMemcpyHostToDevice);
-    }
-    {
-        std::vector<int2> chunks;
-        for (int i = 0; i < n; ++i)
-            for (int64_t f = 0; f < e->host_inst[i].shell_count; f += ENS_FILL_CHUNK)
-                chunks.push_back(make_int2(i, (int)f));
-        e->nfill = (int)chunks.size();
-        if (err == hipSuccess && e->nfill > 0) err = hipMalloc(&e->d_fill, sizeof(int2) * chunks.size());
-        if (err == hipSuccess && e->nfill > 0)
-            err = hipMemcpy(e->d_fill, chunks.data(), sizeof(int2) * chunks.size(), hipMemcpyHostToDevice);
     }
     if (err == hipSuccess) err = hipMalloc(&e->d_queues, 64 * sizeof(int));
     if (err == hipSuccess) err = hipMemset(e->d_queues, 0, 64 * sizeof(int));

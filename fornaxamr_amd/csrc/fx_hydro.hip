@@ -40,11 +40,19 @@ constexpr int NT = 256;
 struct PlaneRegs {
     double own[6];
     double halo[6];
+    int fo, fh;            // sign flips still to apply (see load_plane)
 };
 
+// Loads one x-plane of the block: the thread's own cell and (has_halo) one
+// halo cell, at offsets already mapped onto active cells.  fo / fh: bit a
+// set where the cell stands for a ghost across a reflecting face normal to
+// axis a, whose normal velocity is negated (the values the ghost fill of the
+// reference would hold, src/prj_boundary.c:413-531).  The flips are applied
+// where the values are consumed (flip_plane / store_plane), so that a
+// prefetch does not wait for its loads.
 __device__ __forceinline__ void load_plane(const double* __restrict__ W, int64_t vs,
                                            int64_t off_own, int64_t off_halo, bool has_halo,
-                                           PlaneRegs& r)
+                                           PlaneRegs& r, int fo, int fh)
 {
 #pragma unroll
     for (int q = 0; q < 6; ++q) r.own[q] = W[q * vs + off_own];
@@ -52,11 +60,30 @@ __device__ __forceinline__ void load_plane(const double* __restrict__ W, int64_t
 #pragma unroll
         for (int q = 0; q < 6; ++q) r.halo[q] = W[q * vs + off_halo];
     }
+    r.fo = fo;
+    r.fh = fh;
 }
 
-__device__ __forceinline__ void store_plane(double (*S)[PC], const PlaneRegs& r, int oidx,
+// -v where bit is set (a sign-bit xor: no branch, bit for bit -v)
+__device__ __forceinline__ double negate_if(double v, int bit)
+{
+    return __longlong_as_double(__double_as_longlong(v) ^ ((long long)(bit & 1) << 63));
+}
+
+__device__ __forceinline__ void flip_plane(PlaneRegs& r, bool has_halo)
+{
+    r.own[1] = negate_if(r.own[1], r.fo);
+    if (has_halo) {
+#pragma unroll
+        for (int q = 1; q <= 3; ++q) r.halo[q] = negate_if(r.halo[q], r.fh >> (q - 1));
+    }
+    r.fo = r.fh = 0;
+}
+
+__device__ __forceinline__ void store_plane(double (*S)[PC], PlaneRegs& r, int oidx,
                                             int hidx, bool has_halo)
 {
+    flip_plane(r, has_halo);
 #pragma unroll
     for (int q = 0; q < NV; ++q) S[q][oidx] = r.own[q];
     if (has_halo) {
@@ -161,11 +188,29 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
         hk = c < 2 ? c : c + BS;
     }
     const int hidx = hj * PW + hk;
-    const int64_t col_own = cell_off(g, x0, y0 + j, z0 + k);
-    const int64_t col_halo = cell_off(g, x0, y0 + hj - NG, z0 + hk - NG);
+    // Cells outside the domain are read where they come from (periodic wrap
+    // or mirror, map_axis), so the stage needs no ghost fill: y/z of the halo
+    // cell once per thread, x once per plane (uniform over the workgroup).
+    int hflip = 0, hym = 0, hzm = 0;
+    if (has_halo) {
+        hym = map_axis(g, 1, g.ny, y0 + hj - NG, hflip);
+        hzm = map_axis(g, 2, g.nz, z0 + hk - NG, hflip);
+    }
+    const int64_t col_own = cell_off(g, 0, y0 + j, z0 + k);
+    const int64_t col_halo = cell_off(g, 0, hym, hzm);
 
-    auto own_at = [&](int p) { return col_own + (int64_t)p * g.sx; };
-    auto halo_at = [&](int p) { return col_halo + (int64_t)p * g.sx; };
+    // active planes (p in [0, BS)): the update and W^n
+    auto own_at = [&](int p) { return col_own + (int64_t)(x0 + p) * g.sx; };
+    // plane p of the stencil, mapped; fx receives its x flip bit
+    auto plane_x = [&](int p, int& fx) {
+        fx = 0;
+        return (int64_t)map_axis(g, 0, g.nx, x0 + p, fx) * g.sx;
+    };
+    auto load_at = [&](int p, PlaneRegs& r) {
+        int fx;
+        const int64_t xo = plane_x(p, fx);
+        load_plane(Win, vs, col_own + xo, col_halo + xo, has_halo, r, fx, fx | hflip);
+    };
 
     double Ls[NQ];             // left state of x-face i+1 (rho..ye, p)
     double Fx[6];              // flux through x-face i
@@ -175,20 +220,24 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
     {
         double cmm[NQ], cm[NQ], c0[NQ], c1[NQ];
         PlaneRegs r;
-        load_plane(Win, vs, own_at(-1), halo_at(-1), has_halo, r);
+        load_at(-1, r);
         store_plane(pl[1], r, oidx, hidx, has_halo);        // plane -1 (temporary)
         v1p[oidx] = r.own[1];
         if (has_halo) v1p[hidx] = r.halo[1];
 #pragma unroll
         for (int q = 0; q < NV; ++q) cm[q] = r.own[q];
-        load_plane(Win, vs, own_at(0), halo_at(0), has_halo, r);
+        load_at(0, r);
         store_plane(pl[0], r, oidx, hidx, has_halo);        // plane 0
+        int fmm, f1;
+        const int64_t omm = col_own + plane_x(-2, fmm), o1 = col_own + plane_x(1, f1);
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             c0[q] = r.own[q];
-            cmm[q] = Win[q * vs + own_at(-2)];
-            c1[q] = Win[q * vs + own_at(1)];
+            cmm[q] = Win[q * vs + omm];
+            c1[q] = Win[q * vs + o1];
         }
+        cmm[1] = negate_if(cmm[1], fmm);
+        c1[1] = negate_if(c1[1], f1);
         cmm[6] = gm1 * cmm[0] * cmm[4];
         cm[6] = gm1 * cm[0] * cm[4];
         c0[6] = gm1 * c0[0] * c0[4];
@@ -218,9 +267,9 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
         double vf[3];
         hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fx, vf);
         __syncthreads();
-        load_plane(Win, vs, own_at(1), halo_at(1), has_halo, r);
+        load_at(1, r);
         store_plane(pl[1], r, oidx, hidx, has_halo);        // plane 1
-        load_plane(Win, vs, own_at(2), halo_at(2), has_halo, pf);
+        load_at(2, pf);
         __syncthreads();
     }
 
@@ -302,6 +351,7 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
                     c0[q] = pl[B][q][oidx];
                     cp[q] = pf.own[q];
                 }
+                cp[1] = negate_if(cp[1], pf.fo);   // pf is flipped when stored
                 cm[6] = gm1 * cm[0] * cm[4];
                 c0[6] = gm1 * c0[0] * c0[4];
                 cp[6] = gm1 * cp[0] * cp[4];
@@ -460,7 +510,7 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
 
 #pragma unroll
         for (int q = 0; q < 6; ++q) Fx[q] = Fxn[q];
-        if (i + 3 <= BS + 1) load_plane(Win, vs, own_at(i + 3), halo_at(i + 3), has_halo, pf);
+        if (i + 3 <= BS + 1) load_at(i + 3, pf);
         __syncthreads();
     }
 }

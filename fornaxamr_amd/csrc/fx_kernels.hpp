@@ -40,22 +40,12 @@ struct EnsInst {
     Geom g;
     double* W[2];           // stage slots of this instance (padded layout)
     double* dev;            // [0] dt, [1] dt_prev, [2] CFL reduction (bits of a positive double)
-    int64_t shell_begin;    // first index of this instance in the batched ghost-shell enumeration
-    int64_t shell_count;
 };
 
 struct EnsBlock {
     int inst, bx, by, bz;
 };
 
-// number of ghost-shell cells of one mesh (the domain-boundary fill)
-int64_t shell_cell_count(const Geom& g);
-
-// shell cells filled by one workgroup of the batched ghost fill
-constexpr int ENS_FILL_CHUNK = 1024;
-// chunks[w] = (instance, first shell id) of workgroup w
-hipError_t launch_ens_fill(const EnsInst* inst, const int2* chunks, int nchunks, int slot,
-                           hipStream_t s);
 hipError_t launch_ens_dt(const EnsInst* inst, int ninst, const EnsBlock* blocks, int nblocks,
                          double cfl, double factor, hipStream_t s);
 // persistent_wgs == 0: one workgroup per table entry (XCD remap if xcd_remap);
@@ -66,7 +56,8 @@ hipError_t launch_ens_stage(const EnsInst* inst, const EnsBlock* blocks, int nbl
                             hipStream_t s);
 
 // fill the two-cell ghost shell of the domain (periodic / outflow / reflect;
-// MHD: cell B from the clamped outflow face fields)
+// MHD: cell B from the clamped outflow face fields).  The MHD stage needs it;
+// the hydro stage reads out-of-domain cells at their sources.
 hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s);
 
 hipError_t launch_hydro_stage(const HydroStageArgs& a, int stage, hipStream_t s);

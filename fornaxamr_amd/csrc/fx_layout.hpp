@@ -45,6 +45,23 @@ __host__ __device__ inline int64_t cell_off(const Geom& g, int x, int y, int z)
     return g.origin + (int64_t)x * g.sx + (int64_t)y * g.sy + z;
 }
 
+// Ghost mapping of one axis: the active cell a ghost coordinate c stands for
+// -- periodic wrap (the wrapped neighbour of src/prj_mesh.c:1597-1645) or the
+// mirror copy of the physical boundary (src/prj_boundary.c:413-487: src =
+// -c-1 / 2n-1-c) -- with bit `axis` of flip toggled where a reflecting face
+// negates the normal velocity.  Active coordinates map to themselves.
+// Written with selects only: inside the stage kernels it sits between a
+// prefetch and its use, where a branch would split the schedule.
+__host__ __device__ inline int map_axis(const Geom& g, int axis, int n, int c, int& flip)
+{
+    const bool lo = c < 0, out = lo || c >= n;
+    const int t = lo ? g.bc[2 * axis] : g.bc[2 * axis + 1];
+    const int wrapped = lo ? c + n : c - n;
+    const int mirror = lo ? -c - 1 : 2 * n - 1 - c;
+    flip ^= (out && t == 1) ? 1 << axis : 0;
+    return out ? (t == 3 ? wrapped : mirror) : c;
+}
+
 inline Geom make_geom(int nx, int ny, int nz, const double* xlim, const int* bc, double gamma,
                       int mhd, double eta)
 {

@@ -11,22 +11,6 @@ namespace {
 
 inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// Ghost mapping of one axis: periodic wrap (the wrapped neighbour of
-// src/prj_mesh.c:1597-1645) or the mirror copy of the physical boundary
-// (src/prj_boundary.c:413-487: src = -a-1 / 2N-1-a), reflect negating the
-// normal velocity.
-// cells written at each end of a row by the hydro z-part of the ghost fill
-constexpr int ZEND = 8;
-static_assert(ZEND <= PADZ && ZEND <= BS, "z-end segment must fit the padding and a block");
-
-__device__ __forceinline__ int map_axis(const Geom& g, int axis, int n, int c, int& flip)
-{
-    if (c >= 0 && c < n) return c;
-    int t = c < 0 ? g.bc[2 * axis] : g.bc[2 * axis + 1];
-    if (t == 3) return c < 0 ? c + n : c - n;
-    if (t == 1) flip ^= 1 << axis;
-    return c < 0 ? -c - 1 : 2 * n - 1 - c;
-}
 
 // outflow face-field ghost value: nearest storage face inside the domain
 // (src/prj_boundary.c:902-995)
@@ -75,14 +59,9 @@ __device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
 // One item of the ghost shell, id in [0, shell_cell_count(g)): the x-part
 // (planes x = -2, -1, nx, nx+1 incl. their y/z ghosts) and the y-part (rows
 // y = -2, -1, ny, ny+1 of the interior x range incl. z ghosts) one cell per
-// item, z fastest; the z-part one pair of cells per item, as one 16-byte
-// access per variable (cells and sources are 16-byte aligned: rows start on
-// 128 bytes).  Hydro: the z-part covers the 8 cells z = -8..-1 and
-// nz..nz+7 at each end of an interior (x, y) row -- 4 items per end, adjacent
-// lanes -- so each end is one whole aligned 64-byte segment per variable
-// instead of a partial one; cells beyond the 2 ghost layers lie in the z
-// padding, which nothing reads.  MHD (cell B from the faces): z = -2, -1 and
-// nz, nz+1 only, one item per end.
+// item, z fastest; the z-part the two ghost cells at one end of an interior
+// (x, y) row per item.  The MHD stage reads this shell; the hydro stage maps
+// its out-of-domain reads onto their sources instead (fx_hydro.hip).
 // The decomposition uses 32-bit unsigned arithmetic (a shell has < 2^31
 // items): 64-bit division by a run-time divisor is a long software sequence.
 __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restrict__ W,
@@ -108,42 +87,12 @@ __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restric
         return;
     }
     unsigned r = id - nx_part - ny_part;
-    if (g.mhd) {
-        const int side = (int)(r & 1u); r >>= 1;
-        const int y = (int)(r % (unsigned)g.ny);
-        const int x = (int)(r / (unsigned)g.ny);
-        const int z0 = side ? g.nz : -NG;         // ghost cells z0, z0 + 1
-        fill_one(g, W, bf, x, y, z0);
-        fill_one(g, W, bf, x, y, z0 + 1);
-        return;
-    }
-    const int q = (int)(r & 3u);                  // pair within the 8-cell end
-    const int side = (int)((r >> 2) & 1u); r >>= 3;
+    const int side = (int)(r & 1u); r >>= 1;
     const int y = (int)(r % (unsigned)g.ny);
     const int x = (int)(r / (unsigned)g.ny);
-    const int z0 = (side ? g.nz : -ZEND) + 2 * q; // cells z0, z0 + 1
-    int flip = 0, flip1 = 0;       // both cells of the pair lie outside: same flip
-    const int m0 = map_axis(g, 2, g.nz, z0, flip);
-    const int m1 = map_axis(g, 2, g.nz, z0 + 1, flip1);
-    // periodic: sources m0, m0 + 1 in order; mirror: m1, m1 + 1 reversed
-    const bool rev = m1 < m0;
-    const int64_t src = cell_off(g, x, y, rev ? m1 : m0);
-    const int64_t dst = cell_off(g, x, y, z0);
-    const int64_t vs = g.vstride;
-#pragma unroll
-    for (int v = 0; v < 6; ++v) {
-        double2 val = *reinterpret_cast<const double2*>(W + v * vs + src);
-        if (rev) {
-            const double t = val.x;
-            val.x = val.y;
-            val.y = t;
-        }
-        if (v == 3 && (flip & 4)) {
-            val.x = -val.x;
-            val.y = -val.y;
-        }
-        *reinterpret_cast<double2*>(W + v * vs + dst) = val;
-    }
+    const int z0 = side ? g.nz : -NG;             // ghost cells z0, z0 + 1
+    fill_one(g, W, bf, x, y, z0);
+    fill_one(g, W, bf, x, y, z0 + 1);
 }
 
 __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const double* __restrict__ bf,
@@ -152,23 +101,6 @@ __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const doubl
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= total) return;
     fill_shell_cell(g, W, bf, id);
-}
-
-// batched fill over the shells of all instances of an ensemble: workgroup w
-// fills ids [first, first + ENS_FILL_CHUNK) of the shell of instance inst,
-// (inst, first) = chunks[w], so the instance is uniform per workgroup
-__global__ void __launch_bounds__(256) ens_fill_kernel(const EnsInst* __restrict__ inst,
-                                                       const int2* __restrict__ chunks, int slot)
-{
-    const int2 c = chunks[blockIdx.x];
-    const EnsInst& I = inst[c.x];
-    const Geom g = I.g;
-    double* W = I.W[slot];
-    const int64_t end = I.shell_count;
-    for (int r = 0; r < ENS_FILL_CHUNK; r += 256) {
-        const int64_t id = (int64_t)c.y + r + threadIdx.x;
-        if (id < end) fill_shell_cell(g, W, nullptr, id);
-    }
 }
 
 __global__ void dt_init_kernel(unsigned long long* s)
@@ -357,12 +289,12 @@ __global__ void emf_batch_kernel(int n, const double* ef, const double* ec, cons
 
 }  // namespace
 
-int64_t shell_cell_count(const Geom& g)
+// number of items of the ghost-shell enumeration of fill_shell_cell
+static int64_t shell_cell_count(const Geom& g)
 {
-    // items: x-part and y-part cells, z-part cell pairs (hydro: 4 per row end)
+    // items: x-part and y-part cells, z-part cell pairs
     const int64_t nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
-    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp +
-           (g.mhd ? 2 : 2 * (ZEND / 2)) * (int64_t)g.nx * g.ny;
+    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 2 * (int64_t)g.nx * g.ny;
 }
 
 // eSSPRK convex blend of the active cells (src/prj_timeint.c:2382-2417, hydro):
@@ -460,14 +392,6 @@ hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* sc
     hipLaunchKernelGGL(dt_reduce_kernel, dim3(blocks), dim3(256), 0, s, g, W, cfl, acc);
     hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, s, acc, dt_out, dt_prev, factor,
                        scale);
-    return hipGetLastError();
-}
-
-hipError_t launch_ens_fill(const EnsInst* inst, const int2* chunks, int nchunks, int slot,
-                           hipStream_t s)
-{
-    if (nchunks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ens_fill_kernel, dim3(nchunks), dim3(256), 0, s, inst, chunks, slot);
     return hipGetLastError();
 }
 
