@@ -346,14 +346,20 @@ __global__ void __launch_bounds__(NT) cf_emf_kernel(const CfPtrs* __restrict__ t
 
 }  // namespace
 
-std::string cf_build_plan(int nblocks, const fx_cf_block* blocks, int nlinks,
-                          const fx_cf_link* links, int nvar, CfHostPlan& p)
+void cf_build_plan(int nblocks, const fx_cf_block* blocks, int nlinks, const fx_cf_link* links,
+                   int nvar, CfHostPlan& p)
 {
     p.flux_err = build_flux(nblocks, blocks, nlinks, links, nvar, p);
-    if (!p.flux_err.empty()) p.flux.clear(), p.flux_off.assign(1, 0);
+    if (!p.flux_err.empty()) {
+        p.flux.clear();
+        p.flux_off.assign(1, 0);
+    }
     p.emf_err = build_emf(nblocks, blocks, nlinks, links, p);
-    if (!p.emf_err.empty()) p.emf.clear(), p.emf_off.assign(1, 0), p.fid_blocks.clear();
-    return "";
+    if (!p.emf_err.empty()) {
+        p.emf.clear();
+        p.emf_off.assign(1, 0);
+        p.fid_blocks.clear();
+    }
 }
 
 hipError_t launch_cf_flux(const CfPtrs* tab, const CfFlux* recs, int n, int nvar, hipStream_t s)

@@ -143,8 +143,9 @@ struct CfHostPlan {
     std::vector<int> fid_blocks;    // local blocks whose edge fidelity is reset
     std::string flux_err, emf_err;  // why a part is unavailable ("" if usable)
 };
-std::string cf_build_plan(int nblocks, const fx_cf_block* blocks, int nlinks,
-                          const fx_cf_link* links, int nvar, CfHostPlan& p);
+// fills p; a part whose inputs are unusable gets an empty list and a reason
+void cf_build_plan(int nblocks, const fx_cf_block* blocks, int nlinks, const fx_cf_link* links,
+                   int nvar, CfHostPlan& p);
 hipError_t launch_cf_flux(const CfPtrs* tab, const CfFlux* recs, int n, int nvar, hipStream_t s);
 hipError_t launch_cf_fid_init(const CfPtrs* tab, const int* blocks, int n, hipStream_t s);
 hipError_t launch_cf_emf(const CfPtrs* tab, const CfEmf* recs, int n, int* status, hipStream_t s);
