@@ -18,7 +18,9 @@ turbulent boxes with edges drawn from {32,48,64,96,128} and gamma from
 [1.3,1.7] (fornaxamr_amd.problems.ragged_ensemble), advanced by one batched
 launch per stage captured in a hipGraph with the autotuned schedule, against
 one set of launches per instance, and the cost of the discrete adjoint (reverse
-sweep with recompute) relative to the forward on the 64^3 acoustic box.
+sweep with recompute) relative to the forward on the 64^3 acoustic box, and
+the per-block drop-in (fx_flux_update_blocks: prj_flux_update on 4096
+device-resident blocks in the reference's own layouts, one launch).
 ``--workload mhd`` prints the line of the MHD stage instead (magnetized Sedov,
 PLM(MC)+HLLD+upwind CT).
 """
@@ -127,6 +129,75 @@ def run_adjoint(reps=3):
                         "checkpoint every 8 (reverse sweep recomputes each segment)"}
 
 
+# per-block drop-in: algorithmic bytes of one prj_flux_update call on a hydro
+# block in the reference's layouts -- read the W slot (6 x 20^3) and eosvar P,
+# gamma (2 x 20^3), write the 13,056 faces' fluxes (6) and face velocities (3)
+BLOCK_CELLS = 20 ** 3
+BLOCK_FACES = 3 * 17 * 16 * 16
+BYTES_BLOCK = 8.0 * (8 * BLOCK_CELLS + 9 * BLOCK_FACES)
+
+
+def run_per_block(nblocks=4096, reps=10):
+    """fx_flux_update_blocks (prj_flux_update over device-resident blocks in the
+    reference's own WIDX / EIDX / VIDX / VRIDX layouts), hydro, one launch."""
+    import ctypes as C
+    import torch
+    from fornaxamr_amd import _native
+    lib = _native.load()
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    gam = 5.0 / 3.0
+    # W_mhd slab: [var][slot][cell]; slot 0 holds the stage input
+    W = torch.empty((nblocks, 6, 2, BLOCK_CELLS), dtype=torch.float64, device=dev)
+    W[:, 0] = 1.0 + 0.1 * torch.rand((nblocks, 2, BLOCK_CELLS), generator=g, device=dev,
+                                       dtype=torch.float64)
+    W[:, 1:4] = 0.3 * torch.randn((nblocks, 3, 2, BLOCK_CELLS), generator=g, device=dev,
+                                  dtype=torch.float64)
+    W[:, 4] = 1.5 + 0.1 * torch.rand((nblocks, 2, BLOCK_CELLS), generator=g, device=dev,
+                                     dtype=torch.float64)
+    W[:, 5] = 0.1
+    # eosvar (AoSoA, 3 components: P, T, gamma); every entry is positive, which
+    # is all the flux update needs of it here
+    E = torch.empty((nblocks, 3 * BLOCK_CELLS), dtype=torch.float64, device=dev)
+    E.uniform_(0.5, 1.0, generator=g)
+    E.view(nblocks, -1, 3, 4)[:, :, 2, :] = gam
+    F = torch.empty((3, nblocks, 6 * BLOCK_CELLS), dtype=torch.float64, device=dev)
+    V = torch.empty((3, nblocks, 3 * BLOCK_CELLS), dtype=torch.float64, device=dev)
+    descs = (_native.BlockIO * nblocks)()
+    for b in range(nblocks):
+        io = descs[b]
+        io.W = W[b].data_ptr()
+        io.var_stride = 2 * BLOCK_CELLS
+        io.eosvar = E[b].data_ptr()
+        for d in range(3):
+            io.flux[d] = F[d, b].data_ptr()
+            io.v_riemann[d] = V[d, b].data_ptr()
+    raw = torch.frombuffer(bytearray(bytes(descs)), dtype=torch.uint8).to(dev)
+    status = torch.zeros(8, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        _native.check(lib.fx_flux_update_blocks(nblocks, C.c_void_p(raw.data_ptr()), 0,
+                                                C.c_void_p(status.data_ptr()),
+                                                C.c_void_p(stream.cuda_stream)))
+    launch()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        launch()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    gbs = BYTES_BLOCK * nblocks / (ms * 1e-3) / 1e9
+    return {"kernel": "block_flux_kernel<0> (prj_flux_update, reference block layouts)",
+            "blocks": nblocks, "ms_per_launch": ms,
+            "faces_per_s": BLOCK_FACES * nblocks / (ms * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS}}
+
+
 def _cpu_worker(args):
     kind, nsteps = args
     import numpy as np  # noqa: F401
@@ -217,6 +288,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble figure")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost ratio")
+    ap.add_argument("--no-per-block", action="store_true",
+                    help="skip the per-block drop-in figure (fx_flux_update_blocks)")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the batched-ensemble figure (for profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
@@ -288,6 +361,8 @@ def main():
         out["batched"] = run_ensemble(args.steps, args.warmup)
     if not mhd and not args.no_adjoint:
         out["adjoint"] = run_adjoint()
+    if not mhd and not args.no_per_block:
+        out["per_block"] = run_per_block()
     print(json.dumps(out), flush=True)
 
 

@@ -26,7 +26,7 @@ constexpr int NT = 256;
 constexpr int RB = 20;          // reference block edge incl. ghosts (PRJ_BS)
 constexpr int RNG = 2;          // ghost layers
 constexpr int AW = 4;           // AoSoA lane width (PRJ_AOSOA_W)
-constexpr int SPLIT = 8;        // workgroups per (block, direction)
+constexpr int SPLIT = 8;        // workgroups per block of the z-face kernel
 
 __device__ __forceinline__ int lidx(int i, int j, int k)
 {
@@ -124,13 +124,16 @@ __device__ void block_face(const fx_block_io& b, int dir, int i, int j, int k, i
     for (int r = 0; r < 3; ++r) vr[bidx(3, gmap(dir, 1 + r) - 1, i, j, k)] = vloc[r];
 }
 
-// grid: (block, direction, SPLIT); faces enumerated k fastest
-template <int MHD>
-__global__ void __launch_bounds__(NT) block_flux_kernel(const fx_block_io* __restrict__ blocks,
-                                                       int* err)
+// One face per thread, faces enumerated k fastest (coalesced for z-faces,
+// whose stencil runs along the arrays' fastest axis); grid (block, SPLIT).
+// The direction is a template constant so that the descriptor's per-direction
+// pointers are selected at compile time.
+template <int MHD, int DIR>
+__global__ void __launch_bounds__(NT) block_face_kernel(const fx_block_io* __restrict__ blocks,
+                                                        int* err)
 {
+    constexpr int dir = DIR;
     const fx_block_io b = blocks[blockIdx.x];
-    const int dir = blockIdx.y;
     const int t0 = MHD ? -1 : 0;
     const int tn = MHD ? 18 : 16;          // transverse count
     int lo[3], cnt[3];
@@ -140,11 +143,151 @@ __global__ void __launch_bounds__(NT) block_flux_kernel(const fx_block_io* __res
         cnt[a] = a == dir ? 17 : tn;
     }
     const int nf = cnt[0] * cnt[1] * cnt[2];
-    for (int f = blockIdx.z * NT + threadIdx.x; f < nf; f += SPLIT * NT) {
+    for (int f = blockIdx.y * NT + threadIdx.x; f < nf; f += SPLIT * NT) {
         const int k = lo[2] + f % cnt[2];
         const int j = lo[1] + (f / cnt[2]) % cnt[1];
         const int i = lo[0] + f / (cnt[2] * cnt[1]);
         block_face<MHD>(b, dir, i, j, k, err);
+    }
+}
+
+// One pencil of faces per thread: the faces n = 0..16 along `dir` through the
+// transverse position (p1, p2), marched with a sliding window of cells so that
+// every cell of the pencil is loaded once and its MC half-slopes (and, for
+// hydro, its transverse velocity differences) are formed once and used by both
+// of its faces -- the values prj_flux_update forms per face
+// (src/prj_flux.c:373-445, 525-579, 1967-2036, 2149-2521): the same function
+// of the same inputs, hence the same bits.
+template <int MHD>
+struct PencilCell {
+    static constexpr int NHV = MHD ? 9 : 6;
+    double q[NHV + 2];   // rotated primitives (local order), then P and gamma
+    double d[2];         // hydro: min one-sided differences of v_t1, v_t2
+};
+
+// grid: (block, direction 0 or 1); hydro 256 pencils (transverse 0..15), MHD
+// 324 (transverse -1..16, for the CT layer) in a 384-thread workgroup.  The
+// pencil threads of a wave read rows along k, coalesced for x- and y-faces;
+// z-faces, whose pencils would run along k, use block_face_kernel.
+template <int MHD>
+__global__ void __launch_bounds__(MHD ? 384 : 256) block_pencil_kernel(
+    const fx_block_io* __restrict__ blocks, int* err)
+{
+    constexpr int NHV = MHD ? 9 : 6;
+    constexpr int NQ = NHV + 2;
+    constexpr int T = MHD ? 18 : 16;      // pencils per transverse axis
+    constexpr int LO = MHD ? -1 : 0;
+    const int pid = threadIdx.x;
+    if (pid >= T * T) return;
+    const fx_block_io* bp = blocks + blockIdx.x;
+    const int dir = blockIdx.y;
+    const int a1 = (dir + 1) % 3, a2 = (dir + 2) % 3;
+    // this block's arrays; the per-direction pointers are read with the
+    // (uniform) direction as index straight from the descriptor
+    const double* __restrict__ W = bp->W;
+    const int64_t vs = bp->var_stride;
+    const double* __restrict__ eos = bp->eosvar;
+    double* F = bp->flux[dir];
+    double* vr = bp->v_riemann[dir];
+    const double* bfn = MHD ? bp->bf[dir] : nullptr;
+    double* bv1o = MHD ? bp->Bv1[dir] : nullptr;
+    double* bv2o = MHD ? bp->Bv2[dir] : nullptr;
+    // transverse coordinates: the fastest-varying one is the array's fastest
+    // axis where possible (k, except for z-faces, where it is j)
+    const int pf = pid / T + LO, ps = pid % T + LO;
+    auto ijk = [&](int n, int& i, int& j, int& k) {
+        i = dir == 0 ? n : pf;
+        j = dir == 0 ? pf : (dir == 1 ? n : ps);
+        k = dir == 2 ? n : ps;
+    };
+    const int sn = dir == 0 ? RB * RB : (dir == 1 ? RB : 1);
+    const int sa1 = a1 == 0 ? RB * RB : (a1 == 1 ? RB : 1);
+    const int sa2 = a2 == 0 ? RB * RB : (a2 == 1 ? RB : 1);
+    int gvar[NHV];
+#pragma unroll
+    for (int lv = 0; lv < NHV; ++lv) gvar[lv] = gmap(dir, lv);
+
+    auto load = [&](int n, PencilCell<MHD>& c) {
+        int i, j, k;
+        ijk(n, i, j, k);
+        const int o = lidx(i, j, k);
+#pragma unroll
+        for (int lv = 0; lv < NHV; ++lv) c.q[lv] = W[(int64_t)gvar[lv] * vs + o];
+        c.q[NHV] = eos[bidx(3, 0, i, j, k)];
+        c.q[NHV + 1] = eos[bidx(3, 2, i, j, k)];
+        if (!MHD) {
+            // velocity deltas of this cell along the two transverse axes
+            // (src/prj_flux.c:1967-2036): v_t1 = local 2, v_t2 = local 3
+            const double* v1 = W + (int64_t)(V1 + a1) * vs;
+            const double* v2 = W + (int64_t)(V1 + a2) * vs;
+            c.d[0] = dmin_delta(c.q[2] - v1[o - sa1], v1[o + sa1] - c.q[2]);
+            c.d[1] = dmin_delta(c.q[3] - v2[o - sa2], v2[o + sa2] - c.q[3]);
+        }
+    };
+    // MC half-slopes of the middle cell (the B1 slot of MHD is replaced by bn)
+    auto half = [&](const PencilCell<MHD>& m, const PencilCell<MHD>& c, const PencilCell<MHD>& p,
+                    double* h) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) h[q] = (MHD && q == B1) ? 0.0 : mc_half(m.q[q], c.q[q], p.q[q]);
+    };
+
+    PencilCell<MHD> cm, c0, cp;      // cells n-1, n, n+1 at face n
+    double hm[NQ], h0[NQ];           // half-slopes of cells n-1 and n
+    {
+        PencilCell<MHD> cmm;
+        load(-2, cmm);
+        load(-1, cm);
+        load(0, c0);
+        half(cmm, cm, c0, hm);
+    }
+    for (int n = 0; n <= BS; ++n) {
+        load(n + 1, cp);
+        half(cm, c0, cp, h0);
+        double WL[NHV], WR[NHV], Fl[NHV], vloc[3];
+#pragma unroll
+        for (int lv = 0; lv < NHV; ++lv) {
+            WL[lv] = cm.q[lv] + hm[lv];
+            WR[lv] = c0.q[lv] - h0[lv];
+        }
+        const double pL = cm.q[NHV] + hm[NHV], pR = c0.q[NHV] - h0[NHV];
+        const double gL = cm.q[NHV + 1] + hm[NHV + 1], gR = c0.q[NHV + 1] - h0[NHV + 1];
+        int i, j, k;
+        ijk(n, i, j, k);
+        bool ok = true;
+        if (MHD) {
+            const double bn = bfn[face_idx(dir, i, j, k)];
+            WL[B1] = bn;
+            WR[B1] = bn;
+            double bv1 = 0.0, bv2 = 0.0;
+            const int rc = hlld(WL, WR, pL, pR, gL, gR, bn, Fl, vloc, &bv1, &bv2);
+            if (rc != 0) {
+                if (atomicCAS(err, 0, rc) == 0) {
+                    err[1] = dir;
+                    err[2] = i;
+                    err[3] = j;
+                    err[4] = k;
+                }
+                ok = false;
+            } else {
+                const int c = lidx(i, j, k);
+                bv1o[c] = bv1;
+                bv2o[c] = bv2;
+            }
+        } else {
+            const double du = c0.q[1] - cm.q[1];
+            hllc(WL, WR, pL, pR, gL, gR, du, dmin_delta(cm.d[0], c0.d[0]),
+                 dmin_delta(cm.d[1], c0.d[1]), Fl, vloc);
+        }
+        if (ok) {
+#pragma unroll
+            for (int lv = 0; lv < NHV; ++lv) F[bidx(NHV, gvar[lv], i, j, k)] = Fl[lv];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) vr[bidx(3, gvar[1 + r] - 1, i, j, k)] = vloc[r];
+        }
+        cm = c0;
+        c0 = cp;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) hm[q] = h0[q];
     }
 }
 
@@ -172,9 +315,14 @@ hipError_t launch_block_flux(const fx_block_io* blocks_dev, int n, int mhd, int*
                              hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const dim3 grid(n, 3, SPLIT);
-    if (mhd) hipLaunchKernelGGL(block_flux_kernel<1>, grid, dim3(NT), 0, s, blocks_dev, err);
-    else hipLaunchKernelGGL(block_flux_kernel<0>, grid, dim3(NT), 0, s, blocks_dev, err);
+    // x- and y-faces as pencils, z-faces one per thread
+    if (mhd) {
+        hipLaunchKernelGGL(block_pencil_kernel<1>, dim3(n, 2), dim3(384), 0, s, blocks_dev, err);
+        hipLaunchKernelGGL((block_face_kernel<1, 2>), dim3(n, SPLIT), dim3(NT), 0, s, blocks_dev, err);
+    } else {
+        hipLaunchKernelGGL(block_pencil_kernel<0>, dim3(n, 2), dim3(256), 0, s, blocks_dev, err);
+        hipLaunchKernelGGL((block_face_kernel<0, 2>), dim3(n, SPLIT), dim3(NT), 0, s, blocks_dev, err);
+    }
     return hipGetLastError();
 }
 
