@@ -27,7 +27,7 @@ struct fx_mesh {
     double* bf[3] = {nullptr, nullptr, nullptr};   // [slot 0, slot 1, spare]
     double* U1 = nullptr;
     double* fdat = nullptr;
-    double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] reduction scratch, [3] fx_calc_dt,
+    double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] (unused), [3] fx_calc_dt,
                              // [4] eSSPRK substep dt
     int* err = nullptr;      // [code, dir, x, y, z, stage]
     int integrator = FX_RK2;
@@ -270,7 +270,8 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
         if (e == hipSuccess) e = hipMalloc(&m->U1, 6 * m->cell_bytes);
         if (e == hipSuccess) e = hipMalloc(&m->fdat, 9 * m->cell_bytes);
     }
-    if (e == hipSuccess) e = hipMalloc(&m->dev, 8 * sizeof(double));
+    // dev[0..7] scalars, then the CFL reduction's partials
+    if (e == hipSuccess) e = hipMalloc(&m->dev, (8 + (size_t)CALC_DT_WGS) * sizeof(double));
     if (e == hipSuccess) e = hipMemset(m->dev, 0, 8 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&m->err, 8 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(m->err, 0, 8 * sizeof(int));
@@ -360,7 +361,7 @@ int fx_mesh_get_bf(fx_mesh* m, int slot, double* bx, double* by, double* bz)
 int fx_calc_dt(fx_mesh* m, double cfl, double* dt)
 {
     if (int rc = check_mesh(m)) return rc;
-    FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 2, m->dev + 3, nullptr, 1.0, m->stream));
+    FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 3, nullptr, 1.0, m->stream));
     double h = 0.0;
     FX_HIP(hipMemcpyAsync(&h, m->dev + 3, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     FX_HIP(hipStreamSynchronize(m->stream));
@@ -430,7 +431,7 @@ int fx_step_async(fx_mesh* m, double cfl, double dt_factor)
 {
     if (int rc = check_mesh(m)) return rc;
     const bool ssp = m->integrator == FX_ESSPRK10_4;
-    FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 2, m->dev + 0, m->dev + 1, dt_factor,
+    FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 0, m->dev + 1, dt_factor,
                           m->stream, ssp ? 6.0 : 1.0));
     if (ssp) return essprk10_4_step(m);
     if (int rc = run_stage(m, 1, m->dev, 0.0)) return rc;
@@ -547,6 +548,7 @@ struct fx_ensemble {
     EnsBlock* d_blocks[2] = {nullptr, nullptr};
     int* d_queues = nullptr;
     double* dev = nullptr;               // 4 doubles per instance
+    double* partial = nullptr;           // CFL partials of the per-instance schedule
     int nblocks = 0;
     hipStream_t stream = nullptr;        // owned: graph capture needs a non-null stream
     // schedule: mode 0 = one launch set per instance, 1 = batched
@@ -577,7 +579,7 @@ int ens_enqueue_step(fx_ensemble* e, double cfl, double factor)
     if (e->mode == 0) {
         for (int i = 0; i < e->n; ++i) {
             const EnsInst& I = e->host_inst[i];
-            FX_HIP(launch_calc_dt(I.g, I.W[0], cfl, I.dev + 2, I.dev + 0, I.dev + 1, factor, s));
+            FX_HIP(launch_calc_dt(I.g, I.W[0], cfl, e->partial, I.dev + 0, I.dev + 1, factor, s));
             for (int st = 1; st <= 2; ++st) {
                 double* Win = st == 1 ? I.W[0] : I.W[1];
                 HydroStageArgs a;
@@ -618,6 +620,7 @@ int fx_ensemble_destroy(fx_ensemble* e)
     (void)hipFree(e->d_blocks[1]);
     (void)hipFree(e->d_queues);
     (void)hipFree(e->dev);
+    (void)hipFree(e->partial);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return FX_OK;
@@ -663,6 +666,7 @@ int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
                 for (int bz = 0; bz < g.nbz; ++bz) e->order[0].push_back({i, bx, by, bz});
     }
     e->nblocks = (int)e->order[0].size();
+    if (err == hipSuccess) err = hipMalloc(&e->partial, (size_t)CALC_DT_WGS * sizeof(double));
     // interleaved table: instances dealt round robin block by block, so that
     // every part of the table mixes all instances
     {

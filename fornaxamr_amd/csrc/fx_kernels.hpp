@@ -39,7 +39,7 @@ struct MhdStageArgs {
 struct EnsInst {
     Geom g;
     double* W[2];           // stage slots of this instance (padded layout)
-    double* dev;            // [0] dt, [1] dt_prev, [2] CFL reduction (bits of a positive double)
+    double* dev;            // [0] dt, [1] dt_prev, [2] CFL reduction (bits of the largest denominator)
 };
 
 struct EnsBlock {
@@ -67,11 +67,13 @@ hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s);
 // cell-centred B of the active cells from the faces (prj_mhd_bf2bc)
 hipError_t launch_bf2bc(const Geom& g, double* W, const double* bf, hipStream_t s);
 
-// CFL time step: dt_out[0] = min over active cells of cfl / denom; if
-// dt_prev is non-null and *dt_prev > 0, also applies min(dt, factor * dt_prev)
-// and stores the result to *dt_prev.
-// CFL dt (times `scale`, the integrator's SSP coefficient), growth-limited
-hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* scratch,
+// CFL time step: dt_out[0] = min(1e99, min over active cells of cfl / denom)
+// times `scale` (the integrator's SSP coefficient); if dt_prev is non-null
+// and *dt_prev > 0, also applies min(dt, factor * dt_prev) and stores the
+// result to *dt_prev.  partial: device scratch of CALC_DT_WGS doubles (one per
+// workgroup of the reduction).
+constexpr int CALC_DT_WGS = 4096;
+hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* partial,
                           double* dt_out, double* dt_prev, double factor, hipStream_t s,
                           double scale = 1.0);
 // eSSPRK convex blend of the active cells: W <- prim(w u(Ws) + (1 - w) u(W))

@@ -2,6 +2,8 @@
 // points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fx_kernels.hpp"
 #include "fx_physics.hpp"
 
@@ -103,46 +105,61 @@ __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const doubl
     fill_shell_cell(g, W, bf, id);
 }
 
-__global__ void dt_init_kernel(unsigned long long* s)
-{
-    s[0] = (unsigned long long)__double_as_longlong(1.0e99);
-}
-
-// CFL denominator of one active cell (src/prj_timeint.c:2163-2182)
-__device__ __forceinline__ double cell_dt(const Geom& g, const double* __restrict__ W, int64_t o,
-                                          double cfl)
+// CFL denominator of one active cell (src/prj_timeint.c:2163-2182).  The
+// reference takes min(1e99, min over cells of cfl / denom); correctly rounded
+// division by a positive number is monotone, so that is min(1e99, cfl / max
+// denom) exactly: the reductions carry the largest denominator (NaN never
+// wins a comparison, as in the reference) and divide once.
+__device__ __forceinline__ double cell_denom(const Geom& g, const double* __restrict__ W,
+                                             int64_t o)
 {
     const double gm1 = g.gamma - 1.0;
     double rho = W[o];
     double p = gm1 * rho * W[4 * g.vstride + o];
     double cs = sqrt(g.gamma * p / rho);
-    double denom = (fabs(W[g.vstride + o]) + cs) / g.dx[0] +
-                   (fabs(W[2 * g.vstride + o]) + cs) / g.dx[1] +
-                   (fabs(W[3 * g.vstride + o]) + cs) / g.dx[2];
-    return cfl / denom;
+    return (fabs(W[g.vstride + o]) + cs) / g.dx[0] + (fabs(W[2 * g.vstride + o]) + cs) / g.dx[1] +
+           (fabs(W[3 * g.vstride + o]) + cs) / g.dx[2];
 }
 
-__device__ __forceinline__ double block_min(double v, double* red)
+__device__ __forceinline__ double dt_from_denom(double cfl, double max_denom)
 {
-    red[threadIdx.x] = v;
+    const double dt = cfl / max_denom;
+    return dt < 1.0e99 ? dt : 1.0e99;
+}
+
+// largest denominator of the 16^3 block (bx, by, bz); thread (j, k) walks x
+__device__ __forceinline__ double block_max_denom(const Geom& g, const double* __restrict__ W,
+                                                  int bx, int by, int bz, double* red)
+{
+    const int j = threadIdx.x >> 4, k = threadIdx.x & 15;
+    const int64_t o0 = cell_off(g, bx * BS, by * BS + j, bz * BS + k);
+    double best = 0.0;
+    for (int i = 0; i < BS; ++i) {
+        const double d = cell_denom(g, W, o0 + (int64_t)i * g.sx);
+        if (d > best) best = d;
+    }
+    red[threadIdx.x] = best;
     __syncthreads();
     for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
-            double o = red[threadIdx.x + w];
-            if (o < red[threadIdx.x]) red[threadIdx.x] = o;
+            const double o = red[threadIdx.x + w];
+            if (o > red[threadIdx.x]) red[threadIdx.x] = o;
         }
         __syncthreads();
     }
     return red[0];
 }
 
-// src/prj_timeint.c:2144-2256 (sound speed only, also for MHD)
+// src/prj_timeint.c:2144-2256 (sound speed only, also for MHD), in two
+// launches without atomics: each workgroup streams a stretch of the cells in
+// storage order (x, y, z) and writes its largest denominator to
+// partial[blockIdx.x]; one workgroup then reduces the partials and forms dt
 __global__ void __launch_bounds__(256) dt_reduce_kernel(Geom g, const double* __restrict__ W,
-                                                        double cfl, unsigned long long* s)
+                                                        double* __restrict__ partial)
 {
     __shared__ double red[256];
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    double best = 1.0e99;
+    double best = 0.0;
     for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
          id += (int64_t)gridDim.x * blockDim.x) {
         const unsigned u = (unsigned)id;     // cells < 2^31
@@ -150,23 +167,25 @@ __global__ void __launch_bounds__(256) dt_reduce_kernel(Geom g, const double* __
         unsigned r = u / (unsigned)g.nz;
         int y = (int)(r % (unsigned)g.ny);
         int x = (int)(r / (unsigned)g.ny);
-        double dt_cell = cell_dt(g, W, cell_off(g, x, y, z), cfl);
-        if (dt_cell < best) best = dt_cell;
+        const double d = cell_denom(g, W, cell_off(g, x, y, z));
+        if (d > best) best = d;
     }
-    best = block_min(best, red);
-    if (threadIdx.x == 0) {
-        // dt > 0: IEEE order of positive doubles is the order of their bits
-        atomicMin(s, (unsigned long long)__double_as_longlong(best));
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x])
+            red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 // dt = min(dt_new, factor * dt_prev) after the first step (src/main.c:517-522);
 // dt_new is the CFL minimum times the integrator's SSP scale (1 for RK2, 6 for
 // eSSPRK(10,4): src/prj_timeint.c:2246-2253)
-__device__ __forceinline__ void dt_finalize(const unsigned long long* s, double* dt_out,
-                                            double* dt_prev, double factor, double scale = 1.0)
+__device__ __forceinline__ void dt_finalize(double dt_new, double* dt_out, double* dt_prev,
+                                            double factor, double scale)
 {
-    double dt_new = __longlong_as_double((long long)s[0]);
     if (scale != 1.0) dt_new *= scale;
     double dt = dt_new;
     if (dt_prev != nullptr) {
@@ -180,47 +199,51 @@ __device__ __forceinline__ void dt_finalize(const unsigned long long* s, double*
     dt_out[0] = dt;
 }
 
-__global__ void dt_finalize_kernel(const unsigned long long* s, double* dt_out, double* dt_prev,
-                                   double factor, double scale)
+__global__ void __launch_bounds__(256) dt_final_kernel(const double* __restrict__ partial, int n,
+                                                       double cfl, double* dt_out, double* dt_prev,
+                                                       double factor, double scale)
 {
-    dt_finalize(s, dt_out, dt_prev, factor, scale);
+    __shared__ double red[256];
+    double best = 0.0;
+    for (int b = threadIdx.x; b < n; b += blockDim.x)
+        if (partial[b] > best) best = partial[b];
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x])
+            red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dt_finalize(dt_from_denom(cfl, red[0]), dt_out, dt_prev, factor, scale);
 }
 
+// ensembles: dev[2] of each instance accumulates the bits of its largest
+// denominator (non-negative doubles order as their bit patterns)
 __global__ void ens_dt_init_kernel(const EnsInst* __restrict__ inst, int ninst)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ninst)
-        inst[i].dev[2] = __longlong_as_double(__double_as_longlong(1.0e99));
+    if (i < ninst) inst[i].dev[2] = 0.0;
 }
 
 // one workgroup per block of the table: 256 threads x 16 cells along x
 __global__ void __launch_bounds__(256) ens_dt_reduce_kernel(const EnsInst* __restrict__ inst,
-                                                            const EnsBlock* __restrict__ blocks,
-                                                            double cfl)
+                                                            const EnsBlock* __restrict__ blocks)
 {
     __shared__ double red[256];
     const EnsBlock e = blocks[blockIdx.x];
-    const Geom g = inst[e.inst].g;
-    const double* W = inst[e.inst].W[0];
-    const int j = threadIdx.x >> 4, k = threadIdx.x & 15;
-    double best = 1.0e99;
-    for (int i = 0; i < BS; ++i) {
-        int64_t o = cell_off(g, e.bx * BS + i, e.by * BS + j, e.bz * BS + k);
-        double d = cell_dt(g, W, o, cfl);
-        if (d < best) best = d;
-    }
-    best = block_min(best, red);
+    const double m = block_max_denom(inst[e.inst].g, inst[e.inst].W[0], e.bx, e.by, e.bz, red);
     if (threadIdx.x == 0)
-        atomicMin(reinterpret_cast<unsigned long long*>(inst[e.inst].dev + 2),
-                  (unsigned long long)__double_as_longlong(best));
+        atomicMax(reinterpret_cast<unsigned long long*>(inst[e.inst].dev + 2),
+                  (unsigned long long)__double_as_longlong(m));
 }
 
-__global__ void ens_dt_finalize_kernel(const EnsInst* __restrict__ inst, int ninst, double factor)
+__global__ void ens_dt_finalize_kernel(const EnsInst* __restrict__ inst, int ninst, double cfl,
+                                       double factor)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ninst) {
         double* d = inst[i].dev;
-        dt_finalize(reinterpret_cast<const unsigned long long*>(d + 2), d, d + 1, factor);
+        dt_finalize(dt_from_denom(cfl, d[2]), d, d + 1, factor, 1.0);
     }
 }
 
@@ -380,18 +403,15 @@ hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* scratch,
+hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* partial,
                           double* dt_out, double* dt_prev, double factor, hipStream_t s,
                           double scale)
 {
-    unsigned long long* acc = reinterpret_cast<unsigned long long*>(scratch);
-    hipLaunchKernelGGL(dt_init_kernel, dim3(1), dim3(1), 0, s, acc);
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    unsigned blocks = grid_for(n, 256);
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(dt_reduce_kernel, dim3(blocks), dim3(256), 0, s, g, W, cfl, acc);
-    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, s, acc, dt_out, dt_prev, factor,
-                       scale);
+    const int nwg = (int)std::min<int64_t>(CALC_DT_WGS, grid_for(n, 256));
+    hipLaunchKernelGGL(dt_reduce_kernel, dim3(nwg), dim3(256), 0, s, g, W, partial);
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(256), 0, s, partial, nwg, cfl, dt_out,
+                       dt_prev, factor, scale);
     return hipGetLastError();
 }
 
@@ -399,9 +419,9 @@ hipError_t launch_ens_dt(const EnsInst* inst, int ninst, const EnsBlock* blocks,
                          double cfl, double factor, hipStream_t s)
 {
     hipLaunchKernelGGL(ens_dt_init_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst, ninst);
-    hipLaunchKernelGGL(ens_dt_reduce_kernel, dim3(nblocks), dim3(256), 0, s, inst, blocks, cfl);
+    hipLaunchKernelGGL(ens_dt_reduce_kernel, dim3(nblocks), dim3(256), 0, s, inst, blocks);
     hipLaunchKernelGGL(ens_dt_finalize_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst,
-                       ninst, factor);
+                       ninst, cfl, factor);
     return hipGetLastError();
 }
 
