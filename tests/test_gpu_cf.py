@@ -224,3 +224,27 @@ def test_plan_errors(gpu):
     assert lib.fx_cf_plan_create(0, None, 0, None, 6, C.byref(h)) == -1
     assert lib.fx_cf_flux_send(None, None) == -1
     assert lib.fx_cf_emf_send(None, None, None) == -1
+
+
+def test_plan_without_links(gpu):
+    """A mesh with no coarser neighbours (uniform): empty lists, the send calls
+    are no-ops, the fidelity reset still runs over the local blocks."""
+    import torch
+    from fornaxamr_amd.amr import CoarseFinePlan
+    mesh = load_mesh("two_level")
+    emf = to_dev(O.cf_emf_inputs(mesh, 5))
+    fid = to_dev([[np.full(O.RB_EDGES, 7, np.int32) for _ in range(3)]
+                  for _ in range(mesh.nblocks)])
+    plan = CoarseFinePlan(mesh.geo, mesh.local(), [], 9, emf=emf, fidelity=fid)
+    assert plan.info() == {"flux_records": 0, "flux_levels": 0, "emf_records": 0,
+                           "emf_levels": 0}
+    before = from_dev(emf)
+    plan.flux_send()
+    plan.emf_send()
+    torch.cuda.synchronize()
+    assert_same(from_dev(emf), before, "emf untouched")
+    loc = mesh.local()
+    got = from_dev(fid)
+    for b in np.flatnonzero(loc):
+        assert set(np.unique(got[b][0]).tolist()) == {0, 2}
+    plan.close()

@@ -82,3 +82,21 @@ def test_emf_batch(gpu):
     assert rc == 0
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), d["emf"])
+
+
+def test_empty_batches(gpu):
+    """n = 0 on every batched entry point: no launch, FX_OK, outputs untouched;
+    a negative count is refused."""
+    import ctypes as C
+    import torch
+    from fornaxamr_amd import _native
+    lib = _native.load()
+    _native.check(lib.fx_riemann_hllc_batch(0, *([None] * 12)))
+    _native.check(lib.fx_riemann_hlld_batch(0, *([None] * 13)))
+    _native.check(lib.fx_emf_upwind_batch(0, *([None] * 5)))
+    status = torch.zeros(8, dtype=torch.int32, device="cuda")
+    _native.check(lib.fx_flux_update_blocks(0, None, 0, C.c_void_p(status.data_ptr()), None))
+    assert lib.fx_riemann_hllc_batch(-1, *([None] * 12)) == -1
+    assert lib.fx_flux_update_blocks(-1, None, 0, None, None) == -1
+    torch.cuda.synchronize()
+    assert status.cpu().tolist() == [0] * 8

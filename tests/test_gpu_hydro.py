@@ -54,6 +54,18 @@ def test_anisotropic_mesh(gpu):
     assert ulp_diff(gm.get_prim(0), om.W0) == 0
 
 
+def test_one_block_thick_mesh(gpu):
+    """16 cells along x and z: the out-of-domain reads of one block reach both
+    of its faces on those axes (mirror and reflect, x-side inner/outer differ)."""
+    pr = P.random_smooth(n=(16, 32, 16), seed=33, bc=(1, 0, 3, 3, 0, 1))
+    gm, om = _pair(pr)
+    for _ in range(3):
+        dtg = gm.step(pr.cfl)
+        dto = om.step(pr.cfl)
+        assert dtg == dto
+    assert ulp_diff(gm.get_prim(0), om.W0) == 0
+
+
 def test_sedov_c1_20_steps(gpu):
     """C1: Sedov 64^3 (4x4x4 blocks), 20 RK2 steps, every step compared."""
     pr = P.sedov(n=(64, 64, 64))
