@@ -31,6 +31,7 @@ struct fx_mesh {
                              // [4] eSSPRK substep dt
     int* err = nullptr;      // [code, dir, x, y, z, stage]
     int integrator = FX_RK2;
+    int ct_split = 1;        // MHD CT workgroups per block
     double* saved[2] = {nullptr, nullptr};   // eSSPRK(10,4) saved states u^n, u5
     hipStream_t stream = nullptr;
     size_t cell_bytes = 0;   // bytes of one padded variable
@@ -192,6 +193,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
     a.err = m->err;
     a.dt_ptr = dt_ptr;
     a.dt = dt;
+    a.ct_split = m->ct_split;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (m->timing) {
         FX_HIP(hipEventCreate(&e0));
@@ -275,6 +277,12 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
     if (e == hipSuccess) e = hipMemset(m->dev, 0, 8 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&m->err, 8 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(m->err, 0, 8 * sizeof(int));
+    if (e == hipSuccess && desc->mhd) {
+        int dev_id = 0, ncu = 0;
+        e = hipGetDevice(&dev_id);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_id);
+        if (e == hipSuccess) m->ct_split = mhd_ct_split(m->g, ncu);
+    }
     if (e != hipSuccess) {
         fx_mesh_destroy(m);
         return fail(e == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,

@@ -33,6 +33,7 @@ struct MhdStageArgs {
     int* err;               // device error word: [code, dir, x, y, z]
     const double* dt_ptr;
     double dt;
+    int ct_split = 1;       // CT workgroups per block (1, 2 or 4), each BS / ct_split x-planes
 };
 
 // ---- ragged ensembles of independent hydro meshes --------------------------
@@ -63,6 +64,8 @@ hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipS
 hipError_t launch_hydro_stage(const HydroStageArgs& a, int stage, hipStream_t s);
 
 hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s);
+// CT workgroups per block: more on meshes with fewer blocks than CT slots
+int mhd_ct_split(const Geom& g, int ncu);
 
 // cell-centred B of the active cells from the faces (prj_mhd_bf2bc)
 hipError_t launch_bf2bc(const Geom& g, double* W, const double* bf, hipStream_t s);

@@ -490,11 +490,18 @@ __global__ void __launch_bounds__(NT) mhd_ct_kernel(MhdStageArgs a)
     const int t = threadIdx.x;
     const int j = t >> 4;
     const int k = t & 15;
-    const int b = block_of_workgroup(blockIdx.x, gridDim.x);
+    // workgroup -> (x-part, block): a block's x-planes may be split over
+    // ct_split workgroups, each marching np planes from x0 with its own
+    // prologue (the face between two parts is formed by both, from the same
+    // edges: identical values)
+    const int wg = block_of_workgroup(blockIdx.x, gridDim.x);
+    const int np = BS / a.ct_split;
+    const int part = wg % a.ct_split;
+    const int b = wg / a.ct_split;
     const int bz = b % g.nbz;
     const int by = (b / g.nbz) % g.nby;
     const int bx = b / (g.nbz * g.nby);
-    const int x0 = bx * BS, y0 = by * BS, z0 = bz * BS;
+    const int x0 = bx * BS + part * np, y0 = by * BS, z0 = bz * BS;
     const int64_t vs = g.vstride;
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt;
     const double eta0 = g.eta;
@@ -623,7 +630,7 @@ __global__ void __launch_bounds__(NT) mhd_ct_kernel(MhdStageArgs a)
     double bx_lo = ct(0, x0, y0 + j, z0 + k, ez_e[0][(j + 1) * BS + k], ez_e[0][j * BS + k],
                       g.dx[1], ey_e[0][j * EW + k + 1], ey_e[0][j * EW + k], g.dx[2]);
 
-    for (int i = 0; i < BS; ++i) {
+    for (int i = 0; i < np; ++i) {
         const int s0 = i & 1;         // cell slot of plane i
         const int s1 = s0 ^ 1;        // plane i + 1
         const int e0 = i & 1;         // edge slot of x = i
@@ -710,18 +717,29 @@ int64_t ghostface_count(const Geom& g)
 
 }  // namespace
 
+int mhd_ct_split(const Geom& g, int ncu)
+{
+    // the CT kernel holds 4 workgroups per CU (36 KB LDS each)
+    const int64_t nblocks = (int64_t)g.nbx * g.nby * g.nbz;
+    int split = 1;
+    while (split < 4 && nblocks * split * 2 <= 4 * (int64_t)ncu) split *= 2;
+    return split;
+}
+
 hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s)
 {
     const int nblocks = a.g.nbx * a.g.nby * a.g.nbz;
+    if (a.ct_split != 1 && a.ct_split != 2 && a.ct_split != 4) return hipErrorInvalidValue;
+    const int nct = nblocks * a.ct_split;
     const int64_t ngf = ghostface_count(a.g);
     hipLaunchKernelGGL(mhd_ghostface_kernel, dim3((unsigned)((ngf + 255) / 256)), dim3(256), 0, s,
                        a, ngf, stage);
     if (stage == 1) {
         hipLaunchKernelGGL(mhd_flux_kernel<1>, dim3(nblocks), dim3(NT), 0, s, a);
-        hipLaunchKernelGGL(mhd_ct_kernel<1>, dim3(nblocks), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(mhd_ct_kernel<1>, dim3(nct), dim3(NT), 0, s, a);
     } else {
         hipLaunchKernelGGL(mhd_flux_kernel<2>, dim3(nblocks), dim3(NT), 0, s, a);
-        hipLaunchKernelGGL(mhd_ct_kernel<2>, dim3(nblocks), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(mhd_ct_kernel<2>, dim3(nct), dim3(NT), 0, s, a);
     }
     return hipGetLastError();
 }
