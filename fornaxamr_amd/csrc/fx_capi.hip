@@ -28,7 +28,12 @@ struct fx_mesh {
     double* U1 = nullptr;
     double* fdat = nullptr;
     double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] (unused), [3] fx_calc_dt,
-                             // [4] eSSPRK substep dt
+                             // [4] eSSPRK substep dt; from [8] the CFL partials
+    int nblocks = 0;
+    // dev + 8 holds the per-block CFL maxima of the current state W[0], as
+    // recorded by the hydro stage 2 that wrote it; anything else that writes
+    // W[0] (or the partials) clears this
+    bool cfl_ready = false;
     int* err = nullptr;      // [code, dir, x, y, z, stage]
     int integrator = FX_RK2;
     int ct_split = 1;        // MHD CT workgroups per block
@@ -163,13 +168,16 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
         a.Wout = stage == 1 ? m->W[1] : m->W[0];
         a.dt_ptr = dt_ptr;
         a.dt = dt;
+        a.cfl_part = stage == 2 ? m->dev + 8 : nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (m->timing) {
             FX_HIP(hipEventCreate(&e0));
             FX_HIP(hipEventCreate(&e1));
             FX_HIP(hipEventRecord(e0, m->stream));
         }
+        m->cfl_ready = false;
         FX_HIP(launch_hydro_stage(a, stage, m->stream));
+        m->cfl_ready = stage == 2;
         if (m->timing) {
             FX_HIP(hipEventRecord(e1, m->stream));
             m->spans.push_back({e0, e1, stage});
@@ -178,6 +186,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
     }
     // MHD: stage 1 reads face slot 0 and writes slot 1; stage 2 reads slots
     // 0 and 1 and writes the spare buffer, which then becomes slot 0.
+    m->cfl_ready = false;
     double* bf_in = stage == 1 ? m->bf[0] : m->bf[1];
     FX_HIP(launch_boundary_fill(g, Win, bf_in, m->stream));
     MhdStageArgs a;
@@ -272,8 +281,11 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
         if (e == hipSuccess) e = hipMalloc(&m->U1, 6 * m->cell_bytes);
         if (e == hipSuccess) e = hipMalloc(&m->fdat, 9 * m->cell_bytes);
     }
-    // dev[0..7] scalars, then the CFL reduction's partials
-    if (e == hipSuccess) e = hipMalloc(&m->dev, (8 + (size_t)CALC_DT_WGS) * sizeof(double));
+    // dev[0..7] scalars, then the CFL partials: one per workgroup of the
+    // reduction or one per block
+    m->nblocks = m->g.nbx * m->g.nby * m->g.nbz;
+    const size_t npart = (size_t)std::max(CALC_DT_WGS, m->nblocks);
+    if (e == hipSuccess) e = hipMalloc(&m->dev, (8 + npart) * sizeof(double));
     if (e == hipSuccess) e = hipMemset(m->dev, 0, 8 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&m->err, 8 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(m->err, 0, 8 * sizeof(int));
@@ -330,6 +342,7 @@ int fx_mesh_set_prim(fx_mesh* m, const double* W)
 {
     if (int rc = check_mesh(m)) return rc;
     if (W == nullptr) return fail(FX_EINVAL, "null W");
+    m->cfl_ready = false;
     for (int s = 0; s < 2; ++s)
         if (int rc = copy_cells(m, m->W[s], W, nullptr, m->nh)) return rc;
     double zero[2] = {0.0, 0.0};
@@ -369,7 +382,11 @@ int fx_mesh_get_bf(fx_mesh* m, int slot, double* bx, double* by, double* bz)
 int fx_calc_dt(fx_mesh* m, double cfl, double* dt)
 {
     if (int rc = check_mesh(m)) return rc;
-    FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 3, nullptr, 1.0, m->stream));
+    if (m->cfl_ready)
+        FX_HIP(launch_dt_final(m->dev + 8, m->nblocks, cfl, m->dev + 3, nullptr, 1.0, 1.0,
+                               m->stream));
+    else
+        FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 3, nullptr, 1.0, m->stream));
     double h = 0.0;
     FX_HIP(hipMemcpyAsync(&h, m->dev + 3, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     FX_HIP(hipStreamSynchronize(m->stream));
@@ -439,9 +456,19 @@ int fx_step_async(fx_mesh* m, double cfl, double dt_factor)
 {
     if (int rc = check_mesh(m)) return rc;
     const bool ssp = m->integrator == FX_ESSPRK10_4;
-    FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 0, m->dev + 1, dt_factor,
-                          m->stream, ssp ? 6.0 : 1.0));
-    if (ssp) return essprk10_4_step(m);
+    // the CFL maximum of W[0] comes from the stage 2 that wrote it when that
+    // was recorded, else from a reduction over the cells
+    const double scale = ssp ? 6.0 : 1.0;
+    if (m->cfl_ready)
+        FX_HIP(launch_dt_final(m->dev + 8, m->nblocks, cfl, m->dev + 0, m->dev + 1, dt_factor,
+                               scale, m->stream));
+    else
+        FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 0, m->dev + 1, dt_factor,
+                              m->stream, scale));
+    if (ssp) {
+        m->cfl_ready = false;
+        return essprk10_4_step(m);
+    }
     if (int rc = run_stage(m, 1, m->dev, 0.0)) return rc;
     return run_stage(m, 2, m->dev, 0.0);
 }
@@ -499,6 +526,7 @@ int fx_mesh_kernel_time(fx_mesh* m, int stage, double* total_ms, int* launches)
 int fx_mesh_view_get(fx_mesh* m, fx_mesh_view* out)
 {
     if (int rc = check_mesh(m)) return rc;
+    m->cfl_ready = false;   // the caller may write the state through the view
     out->W[0] = m->W[0];
     out->W[1] = m->W[1];
     for (int s = 0; s < 3; ++s) out->bf[s] = m->bf[s];
@@ -556,8 +584,12 @@ struct fx_ensemble {
     EnsBlock* d_blocks[2] = {nullptr, nullptr};
     int* d_queues = nullptr;
     double* dev = nullptr;               // 4 doubles per instance
-    double* partial = nullptr;           // CFL partials of the per-instance schedule
+    double* partial = nullptr;           // per-block CFL maxima of the per-instance schedule
+    std::vector<int> part_off;           // first partial of each instance
     int nblocks = 0;
+    // the CFL maxima of the current states are recorded (dev[2] of every
+    // instance in the batched schedule, `partial` in the per-instance one)
+    bool cfl_ready = false;
     hipStream_t stream = nullptr;        // owned: graph capture needs a non-null stream
     // schedule: mode 0 = one launch set per instance, 1 = batched
     int mode = 1;
@@ -581,13 +613,33 @@ void ens_drop_graph(fx_ensemble* e)
     e->graph = nullptr;
 }
 
+int ens_nblocks(const Geom& g) { return g.nbx * g.nby * g.nbz; }
+
+// CFL maxima of the current states, read from the cells, where the step
+// expects them (a stage 2 records them for the step after it)
+int ens_enqueue_cfl(fx_ensemble* e)
+{
+    hipStream_t s = e->stream;
+    if (e->mode == 0) {
+        for (int i = 0; i < e->n; ++i) {
+            const EnsInst& I = e->host_inst[i];
+            FX_HIP(launch_dt_reduce(I.g, I.W[0], e->partial + e->part_off[i], ens_nblocks(I.g), s));
+        }
+        return FX_OK;
+    }
+    FX_HIP(launch_ens_dt_reduce(e->d_inst, e->n, e->d_blocks[e->table], e->nblocks, s));
+    return FX_OK;
+}
+
+// One RK2 step of every instance; the CFL maxima of the states are recorded.
 int ens_enqueue_step(fx_ensemble* e, double cfl, double factor)
 {
     hipStream_t s = e->stream;
     if (e->mode == 0) {
         for (int i = 0; i < e->n; ++i) {
             const EnsInst& I = e->host_inst[i];
-            FX_HIP(launch_calc_dt(I.g, I.W[0], cfl, e->partial, I.dev + 0, I.dev + 1, factor, s));
+            double* part = e->partial + e->part_off[i];
+            FX_HIP(launch_dt_final(part, ens_nblocks(I.g), cfl, I.dev + 0, I.dev + 1, factor, 1.0, s));
             for (int st = 1; st <= 2; ++st) {
                 double* Win = st == 1 ? I.W[0] : I.W[1];
                 HydroStageArgs a;
@@ -597,16 +649,19 @@ int ens_enqueue_step(fx_ensemble* e, double cfl, double factor)
                 a.Wout = st == 1 ? I.W[1] : I.W[0];
                 a.dt_ptr = I.dev;
                 a.dt = 0.0;
+                a.cfl_part = st == 2 ? part : nullptr;
                 FX_HIP(launch_hydro_stage(a, st, s));
             }
         }
         return FX_OK;
     }
     const EnsBlock* tbl = e->d_blocks[e->table];
-    FX_HIP(launch_ens_dt(e->d_inst, e->n, tbl, e->nblocks, cfl, factor, s));
-    for (int st = 1; st <= 2; ++st)
+    FX_HIP(launch_ens_dt_finalize(e->d_inst, e->n, cfl, factor, s));
+    for (int st = 1; st <= 2; ++st) {
+        if (st == 2) FX_HIP(launch_ens_dt_clear(e->d_inst, e->n, s));
         FX_HIP(launch_ens_stage(e->d_inst, tbl, e->nblocks, st, e->persistent_wgs, e->xcd, e->xcd,
                                 e->d_queues, s));
+    }
     return FX_OK;
 }
 
@@ -674,7 +729,11 @@ int fx_ensemble_create(int n, const fx_mesh_desc* descs, fx_ensemble** out)
                 for (int bz = 0; bz < g.nbz; ++bz) e->order[0].push_back({i, bx, by, bz});
     }
     e->nblocks = (int)e->order[0].size();
-    if (err == hipSuccess) err = hipMalloc(&e->partial, (size_t)CALC_DT_WGS * sizeof(double));
+    for (int i = 0, acc = 0; i < n; ++i) {
+        e->part_off.push_back(acc);
+        acc += ens_nblocks(e->geoms[i]);
+    }
+    if (err == hipSuccess) err = hipMalloc(&e->partial, (size_t)std::max(1, e->nblocks) * sizeof(double));
     // interleaved table: instances dealt round robin block by block, so that
     // every part of the table mixes all instances
     {
@@ -727,6 +786,7 @@ static int ens_check(fx_ensemble* e, int i)
 int fx_ensemble_set_prim(fx_ensemble* e, int i, const double* W)
 {
     if (int rc = ens_check(e, i)) return rc;
+    e->cfl_ready = false;
     fx_mesh tmp;
     tmp.g = e->geoms[i];
     tmp.stream = e->stream;
@@ -758,6 +818,7 @@ int fx_ensemble_set_schedule(fx_ensemble* e, int mode, int persistent_wgs, int x
     e->persistent_wgs = persistent_wgs;
     e->xcd = xcd ? 1 : 0;
     e->table = table;
+    e->cfl_ready = false;   // the schedules record the CFL maxima in different places
     ens_drop_graph(e);
     return FX_OK;
 }
@@ -774,6 +835,10 @@ int fx_ensemble_set_graph(fx_ensemble* e, int on)
 int fx_ensemble_step_async(fx_ensemble* e, double cfl, double dt_factor)
 {
     if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    if (!e->cfl_ready) {
+        if (int rc = ens_enqueue_cfl(e)) return rc;
+        e->cfl_ready = true;
+    }
     if (!e->use_graph) return ens_enqueue_step(e, cfl, dt_factor);
     if (e->exec == nullptr || e->graph_cfl != cfl || e->graph_factor != dt_factor) {
         ens_drop_graph(e);

@@ -148,9 +148,12 @@ struct HydroLds {
 };
 
 // One RK2 stage of block (bx, by, bz) of the mesh described by g.  Called by
-// the single-mesh kernel and by the batched ensemble kernels.
+// the single-mesh kernel and by the batched ensemble kernels.  Stage 2 returns
+// the largest CFL denominator of the cells this thread wrote (the next step's
+// time-step reduction, taken while the new state is in registers); stage 1
+// returns 0.
 template <int STAGE>
-__device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
+__device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
                                             const double* __restrict__ Win,
                                             const double* __restrict__ W0,
                                             double* __restrict__ Wout, double dt, int bx, int by,
@@ -172,6 +175,7 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
     const int64_t vs = g.vstride;
     const double gm1 = g.gamma - 1.0;
     const double gam = g.gamma;
+    double cfl_best = 0.0;
 
     // own cell and (for t < 144) one halo cell of the 20x20 plane
     const int oidx = (j + NG) * PW + (k + NG);
@@ -506,6 +510,10 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
             cons2prim6(u1, w);
 #pragma unroll
             for (int v = 0; v < 6; ++v) __builtin_nontemporal_store(w[v], &Wout[v * vs + o]);
+            if (STAGE == 2) {
+                const double d = cfl_denom(g, w[0], w[1], w[2], w[3], w[4]);
+                if (d > cfl_best) cfl_best = d;   // NaN never wins
+            }
         }
 
 #pragma unroll
@@ -513,6 +521,25 @@ __device__ __forceinline__ void hydro_block(HydroLds& L, const Geom& g,
         if (i + 3 <= BS + 1) load_at(i + 3, pf);
         __syncthreads();
     }
+    return cfl_best;
+}
+
+// Largest of the 256 threads' values (none of them NaN), valid in thread 0.
+// lds4: four doubles of LDS that no thread touches until the next barrier.
+__device__ __forceinline__ double workgroup_max(double v, double* lds4)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        if (o > v) v = o;
+    }
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = lds4[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w)
+        if (lds4[w] > r) r = lds4[w];
+    return r;
 }
 
 template <int STAGE>
@@ -525,7 +552,11 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
     const int by = (b / g.nbz) % g.nby;
     const int bx = b / (g.nbz * g.nby);
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt;
-    hydro_block<STAGE>(L, g, a.Win, a.W0, a.Wout, dt, bx, by, bz);
+    const double d = hydro_block<STAGE>(L, g, a.Win, a.W0, a.Wout, dt, bx, by, bz);
+    if (STAGE == 2 && a.cfl_part != nullptr) {
+        const double m = workgroup_max(d, L.v1p);
+        if (threadIdx.x == 0) a.cfl_part[b] = m;
+    }
 }
 
 // One table entry of a batched stage: block (bx, by, bz) of instance `inst`.
@@ -536,8 +567,17 @@ __device__ __forceinline__ void ens_entry(HydroLds& L, const EnsInst* __restrict
     const EnsBlock e = blocks[b];
     const EnsInst& I = inst[e.inst];
     const double dt = I.dev[0];
-    hydro_block<STAGE>(L, I.g, I.W[STAGE == 1 ? 0 : 1], I.W[0], I.W[STAGE == 1 ? 1 : 0], dt,
-                       e.bx, e.by, e.bz);
+    const double d = hydro_block<STAGE>(L, I.g, I.W[STAGE == 1 ? 0 : 1], I.W[0],
+                                        I.W[STAGE == 1 ? 1 : 0], dt, e.bx, e.by, e.bz);
+    if (STAGE == 2) {
+        // dev[2] holds the bits of the instance's largest denominator
+        // (non-negative doubles order as their bit patterns); the step clears
+        // it before this launch
+        const double m = workgroup_max(d, L.v1p);
+        if (threadIdx.x == 0)
+            atomicMax(reinterpret_cast<unsigned long long*>(I.dev + 2),
+                      (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // Batched stage over a table of (instance, block) entries, one workgroup per
@@ -576,6 +616,11 @@ __global__ void __launch_bounds__(NT) ens_stage_persistent_kernel(const EnsInst*
             __syncthreads();
             if (b >= hi) break;
             ens_entry<STAGE>(L, inst, blocks, b);
+            // Keeps the entry's thread-0 epilogue (the CFL atomic of stage 2)
+            // and the next fetch apart: without it the compiler merges the two
+            // thread-0 regions across the back edge and parks lane 0 outside
+            // the loop while the rest of its wave goes round again (a hang).
+            __syncthreads();
         }
     }
 }

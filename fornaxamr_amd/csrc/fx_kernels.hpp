@@ -18,6 +18,9 @@ struct HydroStageArgs {
     double* Wout;           // stage output (may alias W0 for stage 2)
     const double* dt_ptr;   // device dt, or null to use dt
     double dt;
+    // stage 2: cfl_part[b] = largest CFL denominator of block b's output
+    // (b = (bx * nby + by) * nbz + bz); null: not recorded
+    double* cfl_part = nullptr;
 };
 
 struct MhdStageArgs {
@@ -47,8 +50,16 @@ struct EnsBlock {
     int inst, bx, by, bz;
 };
 
-hipError_t launch_ens_dt(const EnsInst* inst, int ninst, const EnsBlock* blocks, int nblocks,
-                         double cfl, double factor, hipStream_t s);
+// Batched CFL step of an ensemble.  launch_ens_dt_reduce: dev[2] of every
+// instance = the bits of its largest denominator, read from its state.
+// launch_ens_dt_clear: dev[2] = 0 (before a batched stage 2, which raises it
+// to the largest denominator of the state it writes).  launch_ens_dt_finalize:
+// dt of every instance from dev[2] (dev[0]) and the factor limit (dev[1]).
+hipError_t launch_ens_dt_reduce(const EnsInst* inst, int ninst, const EnsBlock* blocks,
+                                int nblocks, hipStream_t s);
+hipError_t launch_ens_dt_clear(const EnsInst* inst, int ninst, hipStream_t s);
+hipError_t launch_ens_dt_finalize(const EnsInst* inst, int ninst, double cfl, double factor,
+                                  hipStream_t s);
 // persistent_wgs == 0: one workgroup per table entry (XCD remap if xcd_remap);
 // otherwise that many persistent workgroups pull table entries from `queues`
 // (8 counters, one per XCD chunk when xcd_queues, else only the first).
@@ -79,6 +90,15 @@ constexpr int CALC_DT_WGS = 4096;
 hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* partial,
                           double* dt_out, double* dt_prev, double factor, hipStream_t s,
                           double scale = 1.0);
+// The two halves of launch_calc_dt: the reduction writes the largest
+// denominator of each of `nwg` stretches of the active cells to partial[0..nwg);
+// the final launch forms dt from n partials -- those of the reduction or the
+// per-block maxima a hydro stage 2 recorded (HydroStageArgs::cfl_part).
+int calc_dt_wgs(const Geom& g);
+hipError_t launch_dt_reduce(const Geom& g, const double* W, double* partial, int nwg,
+                            hipStream_t s);
+hipError_t launch_dt_final(const double* partial, int n, double cfl, double* dt_out,
+                           double* dt_prev, double factor, double scale, hipStream_t s);
 // eSSPRK convex blend of the active cells: W <- prim(w u(Ws) + (1 - w) u(W))
 hipError_t launch_blend(const Geom& g, double* W, const double* Ws, double saved_weight,
                         hipStream_t s);

@@ -34,6 +34,8 @@ struct Geom {
     double vol;
     double inv_vol;             // 1/vol, exact when vol_pow2
     int vol_pow2;               // vol is a power of two: x/vol == x*inv_vol bit for bit
+    double inv_dx[3];           // 1/dx, exact when dx_pow2
+    int dx_pow2;                // every dx is a power of two with a finite reciprocal
     double gamma;
     int bc[6];                  // reference codes: 0 outflow, 1 reflect, 3 periodic
     int mhd;
@@ -62,6 +64,32 @@ __host__ __device__ inline int map_axis(const Geom& g, int axis, int n, int c, i
     return out ? (t == 3 ? wrapped : mirror) : c;
 }
 
+// CFL denominator of one cell from its primitives (src/prj_timeint.c:2163-2182):
+// (|v1| + cs)/dx1 + (|v2| + cs)/dx2 + (|v3| + cs)/dx3 with cs^2 = gamma p / rho
+// and p = (gamma - 1) rho e.  Dividing by a power of two and multiplying by
+// its (exact) reciprocal round the same real number, so the dx_pow2 form is
+// the same double.
+__host__ __device__ inline double cfl_denom(const Geom& g, double rho, double v1, double v2,
+                                            double v3, double e)
+{
+    const double p = (g.gamma - 1.0) * rho * e;
+    const double cs = sqrt(g.gamma * p / rho);
+    const double a = fabs(v1) + cs, b = fabs(v2) + cs, c = fabs(v3) + cs;
+    if (g.dx_pow2) return a * g.inv_dx[0] + b * g.inv_dx[1] + c * g.inv_dx[2];
+    return a / g.dx[0] + b / g.dx[1] + c / g.dx[2];
+}
+
+// exact reciprocal of x when x is a power of two whose reciprocal is a
+// finite double, else 0
+inline double pow2_reciprocal(double x)
+{
+    int e = 0;
+    const double m = std::frexp(x, &e);   // x = m * 2^e, m in [0.5, 1)
+    if (m != 0.5) return 0.0;
+    const double r = std::ldexp(1.0, 1 - e);
+    return std::isfinite(r) ? r : 0.0;
+}
+
 inline Geom make_geom(int nx, int ny, int nz, const double* xlim, const int* bc, double gamma,
                       int mhd, double eta)
 {
@@ -85,10 +113,15 @@ inline Geom make_geom(int nx, int ny, int nz, const double* xlim, const int* bc,
     g.area[1] = g.dx[0] * g.dx[2];
     g.area[2] = g.dx[0] * g.dx[1];
     {
-        int e = 0;
-        double m = std::frexp(g.vol, &e);   // vol = m * 2^e, m in [0.5, 1)
-        g.vol_pow2 = (m == 0.5) ? 1 : 0;
-        g.inv_vol = g.vol_pow2 ? std::ldexp(1.0, 1 - e) : 1.0 / g.vol;
+        const double r = pow2_reciprocal(g.vol);
+        g.vol_pow2 = r != 0.0 ? 1 : 0;
+        g.inv_vol = g.vol_pow2 ? r : 1.0 / g.vol;
+        g.dx_pow2 = 1;
+        for (int d = 0; d < 3; ++d) {
+            const double rd = pow2_reciprocal(g.dx[d]);
+            if (rd == 0.0) g.dx_pow2 = 0;
+            g.inv_dx[d] = rd != 0.0 ? rd : 1.0 / g.dx[d];
+        }
     }
     g.gamma = gamma;
     for (int i = 0; i < 6; ++i) g.bc[i] = bc[i];

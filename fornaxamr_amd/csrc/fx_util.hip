@@ -113,12 +113,8 @@ __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const doubl
 __device__ __forceinline__ double cell_denom(const Geom& g, const double* __restrict__ W,
                                              int64_t o)
 {
-    const double gm1 = g.gamma - 1.0;
-    double rho = W[o];
-    double p = gm1 * rho * W[4 * g.vstride + o];
-    double cs = sqrt(g.gamma * p / rho);
-    return (fabs(W[g.vstride + o]) + cs) / g.dx[0] + (fabs(W[2 * g.vstride + o]) + cs) / g.dx[1] +
-           (fabs(W[3 * g.vstride + o]) + cs) / g.dx[2];
+    const int64_t vs = g.vstride;
+    return cfl_denom(g, W[o], W[vs + o], W[2 * vs + o], W[3 * vs + o], W[4 * vs + o]);
 }
 
 __device__ __forceinline__ double dt_from_denom(double cfl, double max_denom)
@@ -403,23 +399,54 @@ hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* partial,
-                          double* dt_out, double* dt_prev, double factor, hipStream_t s,
-                          double scale)
+int calc_dt_wgs(const Geom& g)
 {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    const int nwg = (int)std::min<int64_t>(CALC_DT_WGS, grid_for(n, 256));
+    return (int)std::min<int64_t>(CALC_DT_WGS, grid_for(n, 256));
+}
+
+hipError_t launch_dt_reduce(const Geom& g, const double* W, double* partial, int nwg,
+                            hipStream_t s)
+{
     hipLaunchKernelGGL(dt_reduce_kernel, dim3(nwg), dim3(256), 0, s, g, W, partial);
-    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(256), 0, s, partial, nwg, cfl, dt_out,
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_final(const double* partial, int n, double cfl, double* dt_out,
+                           double* dt_prev, double factor, double scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(256), 0, s, partial, n, cfl, dt_out,
                        dt_prev, factor, scale);
     return hipGetLastError();
 }
 
-hipError_t launch_ens_dt(const EnsInst* inst, int ninst, const EnsBlock* blocks, int nblocks,
-                         double cfl, double factor, hipStream_t s)
+hipError_t launch_calc_dt(const Geom& g, const double* W, double cfl, double* partial,
+                          double* dt_out, double* dt_prev, double factor, hipStream_t s,
+                          double scale)
+{
+    const int nwg = calc_dt_wgs(g);
+    if (hipError_t e = launch_dt_reduce(g, W, partial, nwg, s)) return e;
+    return launch_dt_final(partial, nwg, cfl, dt_out, dt_prev, factor, scale, s);
+}
+
+hipError_t launch_ens_dt_reduce(const EnsInst* inst, int ninst, const EnsBlock* blocks,
+                                int nblocks, hipStream_t s)
 {
     hipLaunchKernelGGL(ens_dt_init_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst, ninst);
-    hipLaunchKernelGGL(ens_dt_reduce_kernel, dim3(nblocks), dim3(256), 0, s, inst, blocks);
+    if (nblocks > 0)
+        hipLaunchKernelGGL(ens_dt_reduce_kernel, dim3(nblocks), dim3(256), 0, s, inst, blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_dt_clear(const EnsInst* inst, int ninst, hipStream_t s)
+{
+    hipLaunchKernelGGL(ens_dt_init_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst, ninst);
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_dt_finalize(const EnsInst* inst, int ninst, double cfl, double factor,
+                                  hipStream_t s)
+{
     hipLaunchKernelGGL(ens_dt_finalize_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst,
                        ninst, cfl, factor);
     return hipGetLastError();

@@ -70,6 +70,11 @@ class GpuEnsemble:
             W = np.ascontiguousarray(p.W, dtype=np.float64)
             _native.check(self.lib.fx_ensemble_set_prim(self.h, i, W.ctypes.data_as(_dp)))
 
+    def set_state(self, i, W):
+        """State of instance i (both slots) and a fresh dt history for it."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        _native.check(self.lib.fx_ensemble_set_prim(self.h, int(i), W.ctypes.data_as(_dp)))
+
     def set_schedule(self, mode=1, persistent_wgs=0, xcd=1, table=0):
         _native.check(self.lib.fx_ensemble_set_schedule(self.h, int(mode), int(persistent_wgs),
                                                         int(xcd), int(table)))

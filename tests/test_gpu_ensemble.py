@@ -124,6 +124,35 @@ def test_graph_recaptures_on_new_cfl(expected):
     e.close()
 
 
+@pytest.mark.parametrize("sched", [(0, 0, 1, 0), (1, 0, 1, 0)])
+def test_state_set_between_steps(expected, sched):
+    """Each step takes the CFL maxima its predecessor's stage 2 recorded; a
+    state set between steps (here: back to the initial one, then one
+    instance changed) must be reduced afresh."""
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    e = GpuEnsemble(_instances())
+    e.set_schedule(*sched)
+    for _ in range(2):
+        e.step_async(CFL, FACTOR)
+    e.reset()
+    for s in range(STEPS):
+        e.step_async(CFL, FACTOR)
+        dts = e.last_dt()
+        for i, (edts, _) in enumerate(expected):
+            assert dts[i] == edts[s], (sched, i, s)
+    pr = _instances()[3]
+    W2 = pr.W.copy()
+    W2[1:4] += 0.5
+    e.set_state(3, W2)
+    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma)
+    om.set_state(W2)
+    e.step_async(CFL, FACTOR)
+    assert e.last_dt()[3] == om.step(CFL, FACTOR)
+    e.sync()
+    assert ulp_diff(e.get_prim(3), om.W0) == 0
+    e.close()
+
+
 def test_autotune_then_step(expected):
     from fornaxamr_amd.ensemble import GpuEnsemble
     e = GpuEnsemble(_instances())

@@ -92,6 +92,39 @@ def test_periodic_100_steps(gpu):
     assert ulp_diff(W, om.W0) == 0
 
 
+def test_recorded_cfl_maximum(gpu):
+    """A step takes its CFL maximum from the one the previous step's stage 2
+    recorded for the state it wrote.  That dt must be the oracle's, equal the
+    reduction over the cells (forced here by taking a view, which drops the
+    record), and follow a state set between steps.  dx = 3/32 is not a power
+    of two (the dividing form of the denominator); the Sedov tests cover the
+    multiplying one."""
+    pr = P.sedov(n=(32, 32, 32), xlim=(-1.5, 1.5) * 3)
+    gm, om = _pair(pr)
+    for s in range(3):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    recorded = gm.calc_dt(pr.cfl)
+    gm.view()
+    reduced = gm.calc_dt(pr.cfl)
+    assert recorded == reduced == om.calc_dt(pr.cfl)
+    # a different state set between steps: the record must not survive it
+    W2 = om.W0.copy()
+    W2[1:4] *= 3.0
+    gm.set_state(W2)
+    om.set_state(W2)
+    for s in range(2):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    assert ulp_diff(gm.get_prim(0), om.W0) == 0
+    # stage 2 on its own records the maximum too
+    dt = om.calc_dt(pr.cfl)
+    assert gm.calc_dt(pr.cfl) == dt
+    gm.stage(1, dt)
+    om.stage(1, dt)
+    gm.stage(2, dt)
+    om.stage(2, dt)
+    assert gm.calc_dt(pr.cfl) == om.calc_dt(pr.cfl)
+
+
 def test_async_steps_match_sync(gpu):
     pr = P.random_smooth(n=(32, 32, 32), seed=33, bc=(3,) * 6)
     from fornaxamr_amd.mesh import GpuMesh
