@@ -30,9 +30,11 @@ struct fx_mesh {
     double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] (unused), [3] fx_calc_dt,
                              // [4] eSSPRK substep dt; from [8] the CFL partials
     int nblocks = 0;
-    // dev + 8 holds the per-block CFL maxima of the current state W[0], as
-    // recorded by the hydro stage 2 that wrote it; anything else that writes
-    // W[0] (or the partials) clears this
+    // dev + 8 holds cfl_nparts CFL maxima of the current state W[0] (per
+    // block for hydro, per CT workgroup for MHD), as recorded by the stage 2
+    // that wrote it; anything else that writes W[0] (or the partials) clears
+    // cfl_ready
+    int cfl_nparts = 0;
     bool cfl_ready = false;
     int* err = nullptr;      // [code, dir, x, y, z, stage]
     int integrator = FX_RK2;
@@ -203,6 +205,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
     a.dt_ptr = dt_ptr;
     a.dt = dt;
     a.ct_split = m->ct_split;
+    a.cfl_part = stage == 2 ? m->dev + 8 : nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (m->timing) {
         FX_HIP(hipEventCreate(&e0));
@@ -210,6 +213,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
         FX_HIP(hipEventRecord(e0, m->stream));
     }
     FX_HIP(launch_mhd_stage(a, stage, m->stream));
+    m->cfl_ready = stage == 2;
     if (m->timing) {
         FX_HIP(hipEventRecord(e1, m->stream));
         m->spans.push_back({e0, e1, stage});
@@ -281,20 +285,21 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
         if (e == hipSuccess) e = hipMalloc(&m->U1, 6 * m->cell_bytes);
         if (e == hipSuccess) e = hipMalloc(&m->fdat, 9 * m->cell_bytes);
     }
-    // dev[0..7] scalars, then the CFL partials: one per workgroup of the
-    // reduction or one per block
-    m->nblocks = m->g.nbx * m->g.nby * m->g.nbz;
-    const size_t npart = (size_t)std::max(CALC_DT_WGS, m->nblocks);
-    if (e == hipSuccess) e = hipMalloc(&m->dev, (8 + npart) * sizeof(double));
-    if (e == hipSuccess) e = hipMemset(m->dev, 0, 8 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&m->err, 8 * sizeof(int));
-    if (e == hipSuccess) e = hipMemset(m->err, 0, 8 * sizeof(int));
     if (e == hipSuccess && desc->mhd) {
         int dev_id = 0, ncu = 0;
         e = hipGetDevice(&dev_id);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_id);
         if (e == hipSuccess) m->ct_split = mhd_ct_split(m->g, ncu);
     }
+    // dev[0..7] scalars, then the CFL partials: one per workgroup of the
+    // reduction, or the maxima a stage 2 records
+    m->nblocks = m->g.nbx * m->g.nby * m->g.nbz;
+    m->cfl_nparts = m->nblocks * (desc->mhd ? m->ct_split : 1);
+    const size_t npart = (size_t)std::max(CALC_DT_WGS, m->cfl_nparts);
+    if (e == hipSuccess) e = hipMalloc(&m->dev, (8 + npart) * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(m->dev, 0, 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&m->err, 8 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(m->err, 0, 8 * sizeof(int));
     if (e != hipSuccess) {
         fx_mesh_destroy(m);
         return fail(e == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,
@@ -361,6 +366,7 @@ int fx_mesh_set_bf(fx_mesh* m, const double* bx, const double* by, const double*
 {
     if (int rc = check_mesh(m)) return rc;
     if (!m->g.mhd) return fail(FX_EINVAL, "mesh has no face fields");
+    m->cfl_ready = false;
     double* in[3] = {(double*)bx, (double*)by, (double*)bz};
     for (int s = 0; s < 2; ++s)
         if (int rc = copy_faces(m, m->bf[s], in, nullptr)) return rc;
@@ -383,7 +389,7 @@ int fx_calc_dt(fx_mesh* m, double cfl, double* dt)
 {
     if (int rc = check_mesh(m)) return rc;
     if (m->cfl_ready)
-        FX_HIP(launch_dt_final(m->dev + 8, m->nblocks, cfl, m->dev + 3, nullptr, 1.0, 1.0,
+        FX_HIP(launch_dt_final(m->dev + 8, m->cfl_nparts, cfl, m->dev + 3, nullptr, 1.0, 1.0,
                                m->stream));
     else
         FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 3, nullptr, 1.0, m->stream));
@@ -460,7 +466,7 @@ int fx_step_async(fx_mesh* m, double cfl, double dt_factor)
     // was recorded, else from a reduction over the cells
     const double scale = ssp ? 6.0 : 1.0;
     if (m->cfl_ready)
-        FX_HIP(launch_dt_final(m->dev + 8, m->nblocks, cfl, m->dev + 0, m->dev + 1, dt_factor,
+        FX_HIP(launch_dt_final(m->dev + 8, m->cfl_nparts, cfl, m->dev + 0, m->dev + 1, dt_factor,
                                scale, m->stream));
     else
         FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 0, m->dev + 1, dt_factor,

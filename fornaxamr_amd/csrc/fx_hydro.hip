@@ -524,24 +524,6 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
     return cfl_best;
 }
 
-// Largest of the 256 threads' values (none of them NaN), valid in thread 0.
-// lds4: four doubles of LDS that no thread touches until the next barrier.
-__device__ __forceinline__ double workgroup_max(double v, double* lds4)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(v, off);
-        if (o > v) v = o;
-    }
-    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double r = lds4[0];
-#pragma unroll
-    for (int w = 1; w < NT / 64; ++w)
-        if (lds4[w] > r) r = lds4[w];
-    return r;
-}
-
 template <int STAGE>
 __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
 {

@@ -37,6 +37,9 @@ struct MhdStageArgs {
     const double* dt_ptr;
     double dt;
     int ct_split = 1;       // CT workgroups per block (1, 2 or 4), each BS / ct_split x-planes
+    // stage 2: cfl_part[w] = largest CFL denominator of the cells CT workgroup
+    // w wrote (nblocks * ct_split entries); null: not recorded
+    double* cfl_part = nullptr;
 };
 
 // ---- ragged ensembles of independent hydro meshes --------------------------

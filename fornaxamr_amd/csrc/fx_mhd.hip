@@ -474,7 +474,7 @@ __device__ double edge_curl_b(const MhdStageArgs& a, int dir, int i, int j, int 
 }
 
 template <int STAGE>
-__global__ void __launch_bounds__(NT) mhd_ct_kernel(MhdStageArgs a)
+__global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
 {
     // cell EMF components and eta mask of planes (ring of 2), cells -1..16
     __shared__ double ce[2][4][CC];
@@ -630,6 +630,10 @@ __global__ void __launch_bounds__(NT) mhd_ct_kernel(MhdStageArgs a)
     double bx_lo = ct(0, x0, y0 + j, z0 + k, ez_e[0][(j + 1) * BS + k], ez_e[0][j * BS + k],
                       g.dx[1], ey_e[0][j * EW + k + 1], ey_e[0][j * EW + k], g.dx[2]);
 
+    // stage 2: each thread's running CFL maximum, kept in LDS (a register
+    // live across the plane loop would cost the kernel its fourth wave)
+    __shared__ double cfl_acc[NT];
+    if (STAGE == 2) cfl_acc[t] = 0.0;
     for (int i = 0; i < np; ++i) {
         const int s0 = i & 1;         // cell slot of plane i
         const int s1 = s0 ^ 1;        // plane i + 1
@@ -680,8 +684,18 @@ __global__ void __launch_bounds__(NT) mhd_ct_kernel(MhdStageArgs a)
             cons2prim9(u, bc, w);
 #pragma unroll
             for (int v = 0; v < 9; ++v) a.Wout[v * vs + o] = w[v];
+            if (STAGE == 2) {
+                // the next step's CFL maximum, from the new state in registers
+                const double d = cfl_denom(g, w[0], w[1], w[2], w[3], w[4]);
+                if (d > cfl_acc[t]) cfl_acc[t] = d;   // NaN never wins
+            }
         }
         bx_lo = bx_hi;
+    }
+    if (STAGE == 2 && a.cfl_part != nullptr) {
+        __syncthreads();   // last plane's nby reads done
+        const double m = workgroup_max(cfl_acc[t], nby);
+        if (t == 0) a.cfl_part[blockIdx.x] = m;
     }
 }
 

@@ -13,6 +13,24 @@
 
 namespace fx {
 
+// Largest of the workgroup's values (none of them NaN; blockDim.x a multiple
+// of 64), valid in thread 0.  lds: one double per wave of LDS that no thread
+// touches until the next barrier after this call.
+__device__ __forceinline__ double workgroup_max(double v, double* lds)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        if (o > v) v = o;
+    }
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = lds[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+        if (lds[w] > r) r = lds[w];
+    return r;
+}
+
 // primitive / conserved variable slots (src/prj_defs.h:466-494)
 enum { RHO = 0, V1 = 1, V2 = 2, V3 = 3, EINT = 4, YE = 5, B1 = 6, B2 = 7, B3 = 8 };
 enum { MOM1 = 1, MOM2 = 2, MOM3 = 3, ETOT = 4 };

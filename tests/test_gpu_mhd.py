@@ -48,6 +48,28 @@ def test_magnetized_sedov_steps(gpu):
         assert np.array_equal(a, b)
 
 
+def test_recorded_cfl_maximum(gpu):
+    """The CT kernel of stage 2 records the next step's CFL maximum per
+    workgroup (a block's planes split over four workgroups on this small
+    mesh).  It must equal the reduction over the cells (forced by taking a
+    view) and the oracle's, and a state set between steps must drop it."""
+    pr = P.magnetized_sedov(n=(32, 48, 32))
+    gm, om = _pair(pr)
+    for s in range(3):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    recorded = gm.calc_dt(pr.cfl)
+    gm.view()
+    assert recorded == gm.calc_dt(pr.cfl) == om.calc_dt(pr.cfl)
+    W2 = om.W0.copy()
+    W2[1:4] *= 2.0
+    bf = [b.copy() for b in om.bf0]
+    gm.set_state(W2, bf)
+    om.set_state(W2, bf)
+    for s in range(2):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    assert ulp_diff(gm.get_prim(0), om.W0) == 0
+
+
 def test_divb_roundoff_128(gpu):
     """C3: magnetized Sedov at 128^3 (512 blocks): |div B| dx / |B| < 1e-13."""
     pr = P.magnetized_sedov(n=(128, 128, 128))
