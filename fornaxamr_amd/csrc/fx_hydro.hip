@@ -280,6 +280,13 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
     const double a1 = g.area[0], a2 = g.area[1], a3 = g.area[2], vol = g.vol;
     const double inv_vol = g.inv_vol;
     const bool vol_pow2 = g.vol_pow2 != 0;
+    // stage 2: W^n of the own cell of the next plane, loaded an iteration
+    // ahead of the update that blends with it
+    double w0n[6];
+    if (STAGE == 2) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) w0n[q] = W0[q * vs + own_at(0)];
+    }
 
     for (int i = 0; i < BS; ++i) {
         const int A = i & 1;
@@ -496,10 +503,8 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
 #pragma unroll
                 for (int v = 0; v < 6; ++v) u1[v] = u[v] + dt * (0.0 + fd[v]);
             } else {
-                double us[6], w0[6];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) w0[q] = W0[q * vs + o];
-                prim2cons6(w0, u);
+                double us[6];
+                prim2cons6(w0n, u);
                 prim2cons6(wi, us);
 #pragma unroll
                 for (int v = 0; v < 6; ++v) {
@@ -519,6 +524,10 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
 #pragma unroll
         for (int q = 0; q < 6; ++q) Fx[q] = Fxn[q];
         if (i + 3 <= BS + 1) load_at(i + 3, pf);
+        if (STAGE == 2 && i + 1 < BS) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) w0n[q] = W0[q * vs + own_at(i + 1)];
+        }
         __syncthreads();
     }
     return cfl_best;
