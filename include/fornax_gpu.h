@@ -68,7 +68,11 @@ int fx_calc_dt(fx_mesh *m, double cfl, double *dt);
 int fx_stage(fx_mesh *m, int stage, double dt);
 /* One main-loop step with dt kept on the device: CFL dt, growth limit
  * dt <= dt_factor * dt_prev (src/main.c:510-522), stage 1, stage 2.
- * Does not synchronise. */
+ * Does not synchronise.  Stage 2 records the CFL maxima of the state it
+ * writes, and the next fx_step_async / fx_calc_dt forms dt from them (the
+ * same value as a pass over the cells) unless the state was written by other
+ * means since: fx_mesh_set_prim, fx_mesh_set_bf, fx_mesh_view_get and the
+ * eSSPRK integrator drop the record. */
 int fx_step_async(fx_mesh *m, double cfl, double dt_factor);
 /* Same, then synchronises and returns the dt used. */
 int fx_step(fx_mesh *m, double cfl, double dt_factor, double *dt_used);
@@ -96,7 +100,10 @@ int fx_mesh_kernel_time(fx_mesh *m, int stage, double *total_ms, int *launches);
 
 /* Device addresses of the state, for zero-copy interop: slot s, variable v
  * of the cell arrays starts at W[s] + v*vstride; cell (x,y,z) at
- * origin + x*sx + y*sy + z.  Face component d of slot s: bf[s] + d*vstride. */
+ * origin + x*sx + y*sy + z.  Face component d of slot s: bf[s] + d*vstride.
+ * Taking a view drops the recorded CFL maxima (see fx_step_async): a caller
+ * that writes the state through a view after a later step calls
+ * fx_mesh_view_get again before the next step. */
 typedef struct fx_mesh_view {
     double *W[2];
     double *bf[3];
