@@ -136,11 +136,21 @@ __device__ __forceinline__ int block_of_workgroup(int wg, int nwg)
 }
 
 // LDS of one block's stage: planes i and i+1 (rho v1 v2 v3 e ye; pressure is
-// formed from rho*e on use), v1 of plane i-1, the face fluxes that cross a
-// wave boundary, and the per-cell MC half-slopes of plane i along y and z.
+// formed from rho*e on use), v1 of planes i-1 and i (v1 of plane p in
+// v1p[(p + 1) & 1]), the face fluxes that cross a wave boundary, and the
+// per-cell MC half-slopes of plane i along y and z.
+//
+// Two barriers per plane.  B1 (after the slopes): plane i+1 stored, slopes
+// written.  B2 (after the faces): fluxes exchanged, every read of plane i's
+// slot done.  After B2 a wave may store plane i+2 into plane i's slot and run
+// on into the next plane's slopes: the slopes read only plane i+1 and write
+// hy/hz, whose readers (this plane's faces) are all behind B2; the next
+// plane's faces, which read plane i+2 and write the exchange buffers that
+// this plane's divergence reads, come after the next B1.  v1 of plane i is
+// saved before B2 into the v1p buffer that held plane i-2.
 struct HydroLds {
     double pl[2][NV][PC];
-    double v1p[PC];
+    double v1p[2][PC];
     double exup[6][4][BS];   // Fy(4(w+1), k) for the last row of wave w
     double ez16[6][BS];      // Fz(j, 16): the block's high-side z faces
     double hy[NQ][288];      // MC half-slopes along y, cells j=-1..16, k=0..15
@@ -226,8 +236,8 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
         PlaneRegs r;
         load_at(-1, r);
         store_plane(pl[1], r, oidx, hidx, has_halo);        // plane -1 (temporary)
-        v1p[oidx] = r.own[1];
-        if (has_halo) v1p[hidx] = r.halo[1];
+        v1p[0][oidx] = r.own[1];                            // plane -1
+        if (has_halo) v1p[0][hidx] = r.halo[1];
 #pragma unroll
         for (int q = 0; q < NV; ++q) cm[q] = r.own[q];
         load_at(0, r);
@@ -418,7 +428,7 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
                 // velocity deltas (src/prj_flux.c:1967-2036)
                 const double* Vn = pl[A][qn];
                 du = Vn[c] - Vn[cl];
-                const double* Xm = v1p;
+                const double* Xm = v1p[i & 1];
                 const double* Xc = pl[A][1];
                 const double* Xp = pl[B][1];
                 const double* Vt = pl[A][dir == 1 ? 3 : 2];
@@ -465,11 +475,14 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
                 for (int v = 0; v < 6; ++v) Fxn[v] = G[v];
             }
         }
-        // own cell of plane i for the update (slot A is overwritten below)
+        // own cell of plane i for the update, and v1 of plane i for the next
+        // plane's faces (slot A is overwritten after B2)
         double wi[6];
 #pragma unroll
         for (int q = 0; q < NV; ++q) wi[q] = pl[A][q][oidx];
-        __syncthreads();   // cross-wave fluxes written; plane i, v1p, slope reads done
+        v1p[(i + 1) & 1][oidx] = wi[1];
+        if (has_halo) v1p[(i + 1) & 1][hidx] = pl[A][1][hidx];
+        __syncthreads();   // B2: cross-wave fluxes written; reads of slot A, v1p, slopes done
 
         // ---- divergence in the reference's x, y, z order ---------------------
         double fs[6];
@@ -489,9 +502,6 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
 #pragma unroll
             for (int v = 0; v < 6; ++v) fd[v] = fs[v] / vol;
         }
-        v1p[oidx] = pl[A][1][oidx];
-        if (has_halo) v1p[hidx] = pl[A][1][hidx];
-        __syncthreads();   // v1 of plane i saved, exchange reads done: slot A is free
         if (i < BS - 1) store_plane(pl[A], pf, oidx, hidx, has_halo);
 
         // ---- per-cell RK2 update -------------------------------------------
@@ -528,7 +538,6 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
 #pragma unroll
             for (int q = 0; q < 6; ++q) w0n[q] = W0[q * vs + own_at(i + 1)];
         }
-        __syncthreads();
     }
     return cfl_best;
 }
@@ -545,7 +554,7 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt;
     const double d = hydro_block<STAGE>(L, g, a.Win, a.W0, a.Wout, dt, bx, by, bz);
     if (STAGE == 2 && a.cfl_part != nullptr) {
-        const double m = workgroup_max(d, L.v1p);
+        const double m = workgroup_max(d, L.v1p[0]);
         if (threadIdx.x == 0) a.cfl_part[b] = m;
     }
 }
@@ -564,7 +573,7 @@ __device__ __forceinline__ void ens_entry(HydroLds& L, const EnsInst* __restrict
         // dev[2] holds the bits of the instance's largest denominator
         // (non-negative doubles order as their bit patterns); the step clears
         // it before this launch
-        const double m = workgroup_max(d, L.v1p);
+        const double m = workgroup_max(d, L.v1p[0]);
         if (threadIdx.x == 0)
             atomicMax(reinterpret_cast<unsigned long long*>(I.dev + 2),
                       (unsigned long long)__double_as_longlong(m));
