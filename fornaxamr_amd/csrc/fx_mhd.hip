@@ -160,9 +160,15 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
 {
     __shared__ double pl[2][NQ][PC];
     // y-face fluxes that cross a wave boundary: rows 4, 8, 12 (first rows of
-    // waves 1-3) and 16 (the block's high-side faces); z-face fluxes at k = 16
-    __shared__ double fyb[6][5][BS];
-    __shared__ double fz16[6][BS];
+    // waves 1-3) and 16 (the block's high-side faces); z-face fluxes at k = 16.
+    // Double-buffered by plane parity, so one barrier per plane suffices: after
+    // plane i's barrier a wave finishes plane i (divergence, update of its own
+    // cell, plane i+2 into plane i's slot -- its own cell and halo cell, which
+    // no other thread reads before the next barrier) and may run on into plane
+    // i+1's faces, which read plane i+1's slot and write the other exchange
+    // buffer while slower waves still read this one.
+    __shared__ double fyb[2][6][5][BS];
+    __shared__ double fz16[2][6][BS];
 
     const Geom& g = a.g;
     const int t = threadIdx.x;
@@ -265,7 +271,7 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
     const double* bfz = a.bf_in + 2 * g.fvstride;
 
     for (int i = 0; i < BS; ++i) {
-        const int A = i & 1;       // LDS slot of plane i
+        const int A = i & 1;       // LDS slot of plane i, and its exchange buffers
         const int B = A ^ 1;       // plane i + 1
         // ---- y/z faces of plane i ------------------------------------------
         const int xw = i & 3;
@@ -295,17 +301,17 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
                 for (int v = 0; v < 6; ++v) Fy[v] = F6[v];
                 if ((j & 3) == 0 && j > 0) {
 #pragma unroll
-                    for (int v = 0; v < 6; ++v) fyb[v][j >> 2][k] = F6[v];
+                    for (int v = 0; v < 6; ++v) fyb[A][v][j >> 2][k] = F6[v];
                 }
             } else if (f == 1) {
 #pragma unroll
                 for (int v = 0; v < 6; ++v) Fz[v] = F6[v];
             } else if (dir == 1) {
 #pragma unroll
-                for (int v = 0; v < 6; ++v) fyb[v][4][fk] = F6[v];
+                for (int v = 0; v < 6; ++v) fyb[A][v][4][fk] = F6[v];
             } else {
 #pragma unroll
-                for (int v = 0; v < 6; ++v) fz16[v][fj] = F6[v];
+                for (int v = 0; v < 6; ++v) fz16[A][v][fj] = F6[v];
             }
         }
         // ---- x-face i+1 from cells i, i+1 (LDS) and i+2 (loaded) ------------
@@ -329,8 +335,8 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
         for (int v = 0; v < 6; ++v) {
             const double fy_dn = __shfl_down(Fy[v], 16);
             const double fz_dn = __shfl_down(Fz[v], 1);
-            const double fy_hi = (j & 3) != 3 ? fy_dn : fyb[v][(j + 1) >> 2][k];
-            const double fz_hi = k != BS - 1 ? fz_dn : fz16[v][j];
+            const double fy_hi = (j & 3) != 3 ? fy_dn : fyb[A][v][(j + 1) >> 2][k];
+            const double fz_hi = k != BS - 1 ? fz_dn : fz16[A][v][j];
             fd[v] = -(a1 * (Fxn[v] - Fx[v]) + a2 * (fy_hi - Fy[v]) + a3 * (fz_hi - Fz[v])) / vol;
         }
         {
@@ -360,10 +366,8 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
         }
 #pragma unroll
         for (int v = 0; v < 6; ++v) Fx[v] = Fxn[v];
-        __syncthreads();
         // ---- plane i+2 into the slot of plane i ----------------------------
         if (i < BS - 1) store_plane(A, c2, i + 2);
-        __syncthreads();
     }
 }
 
@@ -639,8 +643,9 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
         const int s1 = s0 ^ 1;        // plane i + 1
         const int e0 = i & 1;         // edge slot of x = i
         const int e1 = e0 ^ 1;        // x = i + 1
-        // cell plane i+1 into the slot of plane i-1
-        __syncthreads();
+        // cell plane i+1 into the slot of plane i-1: that slot's last readers
+        // (the edges of the previous plane, or of the prologue) are behind the
+        // barrier that preceded the previous CT step
         cell_fill(s1, i + 1);
         __syncthreads();
         for (int e = t; e < EW * EW; e += NT) {
