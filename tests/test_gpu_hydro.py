@@ -138,15 +138,17 @@ def test_async_steps_match_sync(gpu):
 
 
 def test_conservation_256(gpu):
-    """C2 at full size (256^3, 4096 blocks): a periodic box conserves mass,
-    momentum, energy and Ye to round-off, and the free-stream state is kept."""
+    """C2 at full size (256^3, 4096 blocks, 100 RK2 steps): a periodic box
+    conserves mass, momentum, energy and Ye to round-off, and the free-stream
+    state is kept."""
     pr = P.turbulent_box(n=(256, 256, 256), seed=1)
     from fornaxamr_amd.mesh import GpuMesh
     gm = GpuMesh.from_problem(pr)
     vol = np.prod([(pr.xlim[2 * d + 1] - pr.xlim[2 * d]) / pr.n[d] for d in range(3)])
     t0 = cons_totals(pr.W, vol)
-    for _ in range(3):
-        gm.step(pr.cfl)
+    for _ in range(100):
+        gm.step_async(pr.cfl)
+    gm.sync()
     t1 = cons_totals(gm.get_prim(0), vol)
     scale = np.array([t0[0], t0[0], t0[0], t0[0], t0[4], t0[5]])
     assert np.all(np.abs(t1 - t0) / np.abs(scale) < 1e-12), (t1 - t0) / scale
