@@ -61,9 +61,13 @@ __device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
 // One item of the ghost shell, id in [0, shell_cell_count(g)): the x-part
 // (planes x = -2, -1, nx, nx+1 incl. their y/z ghosts) and the y-part (rows
 // y = -2, -1, ny, ny+1 of the interior x range incl. z ghosts) one cell per
-// item, z fastest; the z-part the two ghost cells at one end of an interior
-// (x, y) row per item.  The MHD stage reads this shell; the hydro stage maps
-// its out-of-domain reads onto their sources instead (fx_hydro.hip).
+// item, z fastest; the z-part the four ghost cells of an interior (x, y) row
+// (z = -2, -1, nz, nz+1) as four adjacent items: one cell per item keeps
+// every item's loads independent (two cells per item serialise, since the
+// stores of the first may alias the loads of the second, and took the 256^3
+// MHD fill from 80 to 154 us).  The MHD stage reads this shell; the hydro
+// stage maps its out-of-domain reads onto their sources instead
+// (fx_hydro.hip).
 // The decomposition uses 32-bit unsigned arithmetic (a shell has < 2^31
 // items): 64-bit division by a run-time divisor is a long software sequence.
 __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restrict__ W,
@@ -89,12 +93,10 @@ __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restric
         return;
     }
     unsigned r = id - nx_part - ny_part;
-    const int side = (int)(r & 1u); r >>= 1;
+    const int c = (int)(r & 3u); r >>= 2;         // 0, 1: z = -2, -1; 2, 3: nz, nz+1
     const int y = (int)(r % (unsigned)g.ny);
     const int x = (int)(r / (unsigned)g.ny);
-    const int z0 = side ? g.nz : -NG;             // ghost cells z0, z0 + 1
-    fill_one(g, W, bf, x, y, z0);
-    fill_one(g, W, bf, x, y, z0 + 1);
+    fill_one(g, W, bf, x, y, c < 2 ? c - NG : g.nz + (c - 2));
 }
 
 __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const double* __restrict__ bf,
@@ -311,9 +313,9 @@ __global__ void emf_batch_kernel(int n, const double* ef, const double* ec, cons
 // number of items of the ghost-shell enumeration of fill_shell_cell
 static int64_t shell_cell_count(const Geom& g)
 {
-    // items: x-part and y-part cells, z-part cell pairs
+    // items: one cell each (x-part, y-part, z-part)
     const int64_t nyp = g.ny + 2 * NG, nzp = g.nz + 2 * NG;
-    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 2 * (int64_t)g.nx * g.ny;
+    return 4 * nyp * nzp + 4 * (int64_t)g.nx * nzp + 4 * (int64_t)g.nx * g.ny;
 }
 
 // eSSPRK convex blend of the active cells (src/prj_timeint.c:2382-2417, hydro):
