@@ -39,6 +39,14 @@ struct fx_mesh {
     int* err = nullptr;      // [code, dir, x, y, z, stage]
     int integrator = FX_RK2;
     int ct_split = 1;        // MHD CT workgroups per block
+    // MHD meshes whose flux kernel fits the chip in one round (at most 2
+    // workgroups per CU): stream and events on which the ghost-layer face
+    // kernel runs beside the flux kernel (default priority, non-blocking).
+    // On larger meshes the flux kernel has every CU busy and the two kernels
+    // run in order (running them together measured 0.4 % slower at 256^3,
+    // 2.7 % faster at 128^3).
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     double* saved[2] = {nullptr, nullptr};   // eSSPRK(10,4) saved states u^n, u5
     hipStream_t stream = nullptr;
     size_t cell_bytes = 0;   // bytes of one padded variable
@@ -212,7 +220,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
         FX_HIP(hipEventCreate(&e1));
         FX_HIP(hipEventRecord(e0, m->stream));
     }
-    FX_HIP(launch_mhd_stage(a, stage, m->stream));
+    FX_HIP(launch_mhd_stage(a, stage, m->stream, m->side, m->fork, m->join));
     m->cfl_ready = stage == 2;
     if (m->timing) {
         FX_HIP(hipEventRecord(e1, m->stream));
@@ -290,6 +298,12 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
         e = hipGetDevice(&dev_id);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_id);
         if (e == hipSuccess) m->ct_split = mhd_ct_split(m->g, ncu);
+        const int64_t nb = (int64_t)m->g.nbx * m->g.nby * m->g.nbz;
+        if (e == hipSuccess && nb <= 2 * (int64_t)ncu) {
+            e = hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&m->join, hipEventDisableTiming);
+        }
     }
     // dev[0..7] scalars, then the CFL partials: one per workgroup of the
     // reduction, or the maxima a stage 2 records
@@ -323,6 +337,7 @@ int fx_mesh_destroy(fx_mesh* m)
     if (m == nullptr) return FX_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     else (void)hipDeviceSynchronize();
+    if (m->side) (void)hipStreamSynchronize(m->side);
     drop_spans(m);
     for (double* p : m->W) (void)hipFree(p);
     for (double* p : m->bf) (void)hipFree(p);
@@ -332,6 +347,9 @@ int fx_mesh_destroy(fx_mesh* m)
     (void)hipFree(m->saved[0]);
     (void)hipFree(m->saved[1]);
     (void)hipFree(m->err);
+    if (m->fork) (void)hipEventDestroy(m->fork);
+    if (m->join) (void)hipEventDestroy(m->join);
+    if (m->side) (void)hipStreamDestroy(m->side);
     delete m;
     return FX_OK;
 }

@@ -77,7 +77,12 @@ hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipS
 
 hipError_t launch_hydro_stage(const HydroStageArgs& a, int stage, hipStream_t s);
 
-hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s);
+// side (optional, with the events fork/join): a second stream on which the
+// ghost-layer face kernel runs beside the flux kernel; the CT kernel waits
+// for both.  side == nullptr: all three kernels in order on s.
+hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s,
+                            hipStream_t side = nullptr, hipEvent_t fork = nullptr,
+                            hipEvent_t join = nullptr);
 // CT workgroups per block: more on meshes with fewer blocks than CT slots
 int mhd_ct_split(const Geom& g, int ncu);
 

@@ -375,33 +375,38 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
 // Face EMF inputs on the transverse ghost layer of the domain: faces of
 // direction d whose transverse coordinates include -1 or N (the reference's
 // -1..16 face loops on boundary blocks, src/prj_flux.c:2241-2251).
-__device__ __forceinline__ bool decode_ghostface(const Geom& g, int64_t id, int& dir, int& x,
+__device__ __forceinline__ bool decode_ghostface(const Geom& g, int64_t id64, int& dir, int& x,
                                                  int& y, int& z)
 {
     // per direction: normal extent N_d + 1, transverse box (N_t + 2) x (N_u + 2)
-    // minus the inner N_t x N_u
+    // minus the inner N_t x N_u.  32-bit unsigned arithmetic: the layer has
+    // about 12 N^2 faces, far below 2^31 for any mesh the C ABI accepts, and
+    // 64-bit division by a run-time divisor is a long software sequence.
     const int n[3] = {g.nx, g.ny, g.nz};
+    unsigned id = (unsigned)id64;
     for (int d = 0; d < 3; ++d) {
-        const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
-        const int64_t outer = (int64_t)(n[t1] + 2) * (n[t2] + 2);
-        const int64_t ring = outer - (int64_t)n[t1] * n[t2];
-        const int64_t cnt = (int64_t)(n[d] + 1) * ring;
+        const int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : (d == 1 ? 0 : 1);
+        const unsigned w2 = (unsigned)(n[t2] + 2);
+        const unsigned outer = (unsigned)(n[t1] + 2) * w2;
+        const unsigned ring = outer - (unsigned)n[t1] * (unsigned)n[t2];
+        const unsigned cnt = (unsigned)(n[d] + 1) * ring;
         if (id < cnt) {
-            const int nd = (int)(id / ring);
-            int64_t r = id % ring;
+            const unsigned nd = id / ring;
+            unsigned r = id - nd * ring;
             int c1, c2;
             // ring enumeration: two full rows (c1 = -1, N) then two side columns
-            const int64_t rows = 2 * (int64_t)(n[t2] + 2);
+            const unsigned rows = 2u * w2;
             if (r < rows) {
-                c1 = r < n[t2] + 2 ? -1 : n[t1];
-                c2 = (int)(r % (n[t2] + 2)) - 1;
+                const unsigned hi = r >= w2 ? 1u : 0u;
+                c1 = hi ? n[t1] : -1;
+                c2 = (int)(r - hi * w2) - 1;
             } else {
                 r -= rows;
                 c1 = (int)(r >> 1);
                 c2 = (r & 1) ? n[t2] : -1;
             }
             int c[3];
-            c[d] = nd;
+            c[d] = (int)nd;
             c[t1] = c1;
             c[t2] = c2;
             dir = d;
@@ -745,21 +750,37 @@ int mhd_ct_split(const Geom& g, int ncu)
     return split;
 }
 
-hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s)
+hipError_t launch_mhd_stage(const MhdStageArgs& a, int stage, hipStream_t s, hipStream_t side,
+                            hipEvent_t fork, hipEvent_t join)
 {
     const int nblocks = a.g.nbx * a.g.nby * a.g.nbz;
     if (a.ct_split != 1 && a.ct_split != 2 && a.ct_split != 4) return hipErrorInvalidValue;
+    if (side != nullptr && (fork == nullptr || join == nullptr)) return hipErrorInvalidValue;
     const int nct = nblocks * a.ct_split;
     const int64_t ngf = ghostface_count(a.g);
-    hipLaunchKernelGGL(mhd_ghostface_kernel, dim3((unsigned)((ngf + 255) / 256)), dim3(256), 0, s,
-                       a, ngf, stage);
-    if (stage == 1) {
-        hipLaunchKernelGGL(mhd_flux_kernel<1>, dim3(nblocks), dim3(NT), 0, s, a);
-        hipLaunchKernelGGL(mhd_ct_kernel<1>, dim3(nct), dim3(NT), 0, s, a);
-    } else {
-        hipLaunchKernelGGL(mhd_flux_kernel<2>, dim3(nblocks), dim3(NT), 0, s, a);
-        hipLaunchKernelGGL(mhd_ct_kernel<2>, dim3(nct), dim3(NT), 0, s, a);
+    if (ngf >= ((int64_t)1 << 31)) return hipErrorInvalidValue;   // 32-bit decode
+    // The ghost-layer faces and the flux kernel's faces are disjoint sets of
+    // fdat entries and both only read the state and the faces: the ghost-layer
+    // kernel (latency-bound, a fraction of the chip) can run beside the flux
+    // kernel (issue-bound at two workgroups per CU).
+    hipStream_t gs = s;
+    if (side != nullptr) {
+        if (hipError_t e = hipEventRecord(fork, s)) return e;
+        if (hipError_t e = hipStreamWaitEvent(side, fork, 0)) return e;
+        gs = side;
     }
+    hipLaunchKernelGGL(mhd_ghostface_kernel, dim3((unsigned)((ngf + 255) / 256)), dim3(256), 0, gs,
+                       a, ngf, stage);
+    if (side != nullptr) {
+        if (hipError_t e = hipEventRecord(join, side)) return e;
+    }
+    if (stage == 1) hipLaunchKernelGGL(mhd_flux_kernel<1>, dim3(nblocks), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL(mhd_flux_kernel<2>, dim3(nblocks), dim3(NT), 0, s, a);
+    if (side != nullptr) {
+        if (hipError_t e = hipStreamWaitEvent(s, join, 0)) return e;
+    }
+    if (stage == 1) hipLaunchKernelGGL(mhd_ct_kernel<1>, dim3(nct), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL(mhd_ct_kernel<2>, dim3(nct), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
