@@ -46,7 +46,10 @@ typedef struct fx_mesh_desc {
  *      single-level mesh: src/prj_mesh.c:1438, src/prj_mesh.c:420) ---------- */
 int fx_mesh_create(const fx_mesh_desc *desc, fx_mesh **out);
 int fx_mesh_destroy(fx_mesh *m);
-/* Stream all launches of this mesh go to (a hipStream_t); NULL = default. */
+/* Stream all launches of this mesh go to (a hipStream_t); NULL = default.
+ * An MHD mesh with at most two blocks per CU also runs its ghost-layer face
+ * kernel on a stream of its own, forked from and joined back into this one
+ * with events within each stage (graph capture follows the fork). */
 int fx_mesh_set_stream(fx_mesh *m, void *hip_stream);
 
 /* ---- state transfer (host arrays in the global layout above) ------------ */
