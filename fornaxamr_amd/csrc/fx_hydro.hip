@@ -181,6 +181,7 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
     const int k = t & 15;
     const int lane = t & 63;
     const int wave = t >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);   // the same, as a scalar
     const int x0 = bx * BS, y0 = by * BS, z0 = bz * BS;
     const int64_t vs = g.vstride;
     const double gm1 = g.gamma - 1.0;
@@ -301,6 +302,9 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
     for (int i = 0; i < BS; ++i) {
         const int A = i & 1;
         const int B = A ^ 1;
+        // the wave with one pass more than the others before the next barrier
+        // (halo slopes here, the extra faces below) issues first on its SIMD
+        if (wv == ((i + 1) & 3)) __builtin_amdgcn_s_setprio(2);
 
         // ---- MC half-slopes of plane i along y and z, once per cell ---------
         // (the reference sweeps each variable once per direction and emits both
@@ -359,6 +363,8 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
         double Fy[6], Fz[6], Fxn[6];
         const int xw = i & 3;
         const bool extra = wave == xw && lane < 32;
+        __builtin_amdgcn_s_setprio(0);
+        if (wv == xw) __builtin_amdgcn_s_setprio(2);
         for (int f = 0; f < 4; ++f) {
             if (f == 2 && !extra) continue;
             double WL[6], WR[6], pL, pR, du, dv, dw;
@@ -483,6 +489,7 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
         v1p[(i + 1) & 1][oidx] = wi[1];
         if (has_halo) v1p[(i + 1) & 1][hidx] = pl[A][1][hidx];
         __syncthreads();   // B2: cross-wave fluxes written; reads of slot A, v1p, slopes done
+        __builtin_amdgcn_s_setprio(0);
 
         // ---- divergence in the reference's x, y, z order ---------------------
         double fs[6];
