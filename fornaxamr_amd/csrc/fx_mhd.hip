@@ -276,6 +276,8 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
         // ---- y/z faces of plane i ------------------------------------------
         const int xw = i & 3;
         const bool extra = (t >> 6) == xw && lane < 32;
+        // the wave with the extra faces issues first on its SIMD until the barrier
+        if (__builtin_amdgcn_readfirstlane(t >> 6) == xw) __builtin_amdgcn_s_setprio(2);
         double Fy[6], Fz[6];
 #pragma unroll 1
         for (int f = 0; f < 3; ++f) {
@@ -328,6 +330,7 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
             x_face(i + 1, c0, c1, c2, Fxn);
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
 
         // ---- divergence (x, y, z order) and the RK combination -------------
         double fd[6];
@@ -653,15 +656,18 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
         // barrier that preceded the previous CT step
         cell_fill(s1, i + 1);
         __syncthreads();
+        // Each loop has a remainder beyond NT items (33, 16, 16); rotating
+        // the thread -> item map by whole waves gives each remainder to a
+        // different wave (0, 3, 2), so no wave carries all three extra passes.
         for (int e = t; e < EW * EW; e += NT) {
             int jj = e / EW, kk = e % EW;
             ex_e[e] = edge_x(i, jj, kk, s0);
         }
-        for (int e = t; e < BS * EW; e += NT) {
+        for (int e = (t + 64) & (NT - 1); e < BS * EW; e += NT) {
             int jj = e / EW, kk = e % EW;
             ey_e[e1][e] = edge_y(i + 1, jj, kk, s0, s1);
         }
-        for (int e = t; e < EW * BS; e += NT) {
+        for (int e = (t + 128) & (NT - 1); e < EW * BS; e += NT) {
             int jj = e / BS, kk = e % BS;
             ez_e[e1][e] = edge_z(i + 1, jj, kk, s0, s1);
         }
@@ -670,12 +676,12 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
         double bx_hi = ct(0, x0 + i + 1, y0 + j, z0 + k, ez_e[e1][(j + 1) * BS + k],
                           ez_e[e1][j * BS + k], g.dx[1], ey_e[e1][j * EW + k + 1],
                           ey_e[e1][j * EW + k], g.dx[2]);
-        for (int f = t; f < EW * BS; f += NT) {
+        for (int f = (t + 64) & (NT - 1); f < EW * BS; f += NT) {   // remainder: wave 3
             int jj = f / BS, kk = f % BS;   // By(i, jj 0..16, kk 0..15)
             nby[f] = ct(1, x0 + i, y0 + jj, z0 + kk, ex_e[jj * EW + kk + 1], ex_e[jj * EW + kk],
                         g.dx[2], ez_e[e1][jj * BS + kk], ez_e[e0][jj * BS + kk], g.dx[0]);
         }
-        for (int f = t; f < BS * EW; f += NT) {
+        for (int f = (t + 128) & (NT - 1); f < BS * EW; f += NT) {  // remainder: wave 2
             int jj = f / EW, kk = f % EW;   // Bz(i, jj 0..15, kk 0..16)
             nbz[f] = ct(2, x0 + i, y0 + jj, z0 + kk, ey_e[e1][jj * EW + kk], ey_e[e0][jj * EW + kk],
                         g.dx[0], ex_e[(jj + 1) * EW + kk], ex_e[jj * EW + kk], g.dx[1]);
