@@ -356,7 +356,7 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
                 }
             }
         }
-        __syncthreads();
+        __syncthreads();   // B1: slopes written, plane i+1 stored
 
         // ---- faces: 0 = own y-face (low side), 1 = own z-face (low side),
         //      2 = extra high-side face (half of one wave), 3 = x-face i+1 ----
