@@ -856,9 +856,8 @@ int fx_ensemble_set_graph(fx_ensemble* e, int on)
     return FX_OK;
 }
 
-int fx_ensemble_step_async(fx_ensemble* e, double cfl, double dt_factor)
+static int ens_step(fx_ensemble* e, double cfl, double dt_factor)
 {
-    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
     if (!e->cfl_ready) {
         if (int rc = ens_enqueue_cfl(e)) return rc;
         e->cfl_ready = true;
@@ -882,6 +881,15 @@ int fx_ensemble_step_async(fx_ensemble* e, double cfl, double dt_factor)
     }
     FX_HIP(hipGraphLaunch(e->exec, e->stream));
     return FX_OK;
+}
+
+int fx_ensemble_step_async(fx_ensemble* e, double cfl, double dt_factor)
+{
+    if (e == nullptr) return fail(FX_EINVAL, "null ensemble");
+    const int rc = ens_step(e, cfl, dt_factor);
+    // a step that failed part-way may have cleared the maxima it was to raise
+    if (rc) e->cfl_ready = false;
+    return rc;
 }
 
 int fx_ensemble_sync(fx_ensemble* e)
