@@ -331,6 +331,13 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
         }
         __syncthreads();
         __builtin_amdgcn_s_setprio(0);
+        // halo cell of plane i+2, loaded here so that its latency overlaps the
+        // divergence and update below (it goes to LDS at the end of the plane)
+        double hh[9];
+        if (has_halo && i < BS - 1) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) hh[q] = a.Win[q * vs + halo_at(i + 2)];
+        }
 
         // ---- divergence (x, y, z order) and the RK combination -------------
         double fd[6];
@@ -370,7 +377,15 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
 #pragma unroll
         for (int v = 0; v < 6; ++v) Fx[v] = Fxn[v];
         // ---- plane i+2 into the slot of plane i ----------------------------
-        if (i < BS - 1) store_plane(A, c2, i + 2);
+        if (i < BS - 1) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) pl[A][q][oidx] = c2[q];
+            if (has_halo) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) pl[A][q][hidx] = hh[q];
+                pl[A][9][hidx] = gm1 * hh[0] * hh[4];
+            }
+        }
     }
 }
 
