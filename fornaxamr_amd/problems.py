@@ -93,8 +93,9 @@ def _lowpass_gaussian(rng, n, kmax):
 
 def turbulent_box(n=(256, 256, 256), seed=1, gamma=5.0 / 3.0, rho0=1.0, p=1.0,
                   amp=0.1, nmodes=8, vrms=0.3, kmax=4, cfl=0.4):
-    """Periodic box: rho = rho0 + amp*sum of 8 seeded sine modes with wavenumbers
-    1..4, uniform pressure, and a smoothed Gaussian random velocity of rms vrms."""
+    """Periodic box: rho = rho0 + amp * (sum of 8 seeded sine modes with integer
+    wavenumbers 1..4 and seeded phases), uniform pressure, and a Gaussian random
+    velocity filtered to |k| <= kmax and scaled to rms vrms."""
     rng = np.random.default_rng(seed)
     xlim = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     (x, y, z), _ = _centres(n, xlim)
@@ -104,7 +105,7 @@ def turbulent_box(n=(256, 256, 256), seed=1, gamma=5.0 / 3.0, rho0=1.0, p=1.0,
         k[rng.integers(0, 3)] = rng.integers(1, kmax + 1)
         phase = rng.uniform(0.0, 2.0 * np.pi)
         rho += amp * np.sin(2.0 * np.pi * (k[0] * x[:, None, None] + k[1] * y[None, :, None] +
-                                           k[2] * z[None, None, :]) + phase) / nmodes * 1.0
+                                           k[2] * z[None, None, :]) + phase)
     v = np.stack([_lowpass_gaussian(rng, tuple(n), kmax) for _ in range(3)])
     v *= vrms / np.sqrt(np.mean(np.sum(v * v, axis=0)))
     W = np.zeros((6,) + tuple(n))
