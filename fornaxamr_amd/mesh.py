@@ -121,6 +121,10 @@ class GpuMesh:
         return ms.value, n.value
 
     def view(self):
+        """Device addresses and strides of the state (fx_mesh_view).  Taking a
+        view drops the CFL maxima the last stage 2 recorded, so the next step
+        reduces over the cells; after writing the state through a view taken
+        before a later step, take the view again before stepping."""
         v = _native.MeshView()
         _native.check(self.lib.fx_mesh_view_get(self.h, C.byref(v)))
         return v
