@@ -112,6 +112,8 @@ def load():
     lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
     lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
     lib.fx_emf_upwind_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+    lib.fx_emf_upwind_block_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.POINTER(vp), _dp,
+                                              C.POINTER(vp), vp, vp, vp]
     _lib = lib
     return lib
 
@@ -128,6 +130,7 @@ EXPORTS = [
     "fx_step_async", "fx_step", "fx_sync", "fx_last_dt", "fx_mesh_view_get",
     "fx_mesh_set_timing", "fx_mesh_kernel_time",
     "fx_riemann_hllc_batch", "fx_riemann_hlld_batch", "fx_emf_upwind_batch",
+    "fx_emf_upwind_block_batch",
     "fx_ensemble_create", "fx_ensemble_destroy", "fx_ensemble_set_prim", "fx_ensemble_get_prim",
     "fx_ensemble_set_schedule", "fx_ensemble_set_graph", "fx_ensemble_step_async",
     "fx_ensemble_sync", "fx_ensemble_last_dt", "fx_ensemble_info", "fx_ensemble_autotune",

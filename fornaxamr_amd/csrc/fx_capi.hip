@@ -595,6 +595,34 @@ int fx_emf_upwind_batch(int n, const double* emf_face, const double* emf_cell,
     return FX_OK;
 }
 
+int fx_emf_upwind_block_batch(int n, const int* edge, const double* emf_face,
+                              const double* emf_cell, const double* v_norm, const double* eta,
+                              const double* const* bf, const double* dx, double* const* block_emf,
+                              double* emf, int* status, void* stream)
+{
+    if (n < 0) return fail(FX_EINVAL, "negative count");
+    if (n == 0) return FX_OK;
+    if (edge == nullptr || emf_face == nullptr || emf_cell == nullptr || v_norm == nullptr ||
+        eta == nullptr || dx == nullptr || emf == nullptr || status == nullptr)
+        return fail(FX_EINVAL, "fx_emf_upwind_block_batch: null array");
+    EmfBlockArgs a{};
+    a.n = n;
+    a.edge = edge;
+    a.ef = emf_face;
+    a.ec = emf_cell;
+    a.vn = v_norm;
+    a.eta = eta;
+    for (int d = 0; d < 3; ++d) {
+        a.bf[d] = bf != nullptr ? bf[d] : nullptr;
+        a.block_emf[d] = block_emf != nullptr ? block_emf[d] : nullptr;
+        a.dx[d] = dx[d];
+    }
+    a.emf = emf;
+    a.status = status;
+    FX_HIP(launch_emf_block_batch(a, (hipStream_t)stream));
+    return FX_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

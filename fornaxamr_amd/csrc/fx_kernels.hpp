@@ -124,6 +124,22 @@ hipError_t launch_hlld_batch(int n, const double* WL, const double* WR, const do
                              double* bv2, int* status, hipStream_t s);
 hipError_t launch_emf_batch(int n, const double* ef, const double* ec, const double* vn,
                             double* emf, hipStream_t s);
+// prj_mhd_emf_upwind with the resistive term over n edges of one block in the
+// reference's block layouts (edge[4m..4m+3] = dir, i, j, k)
+struct EmfBlockArgs {
+    int n;
+    const int* edge;
+    const double* ef;       // emf_face, emf_cell, v_norm: 4 x n (SoA)
+    const double* ec;
+    const double* vn;
+    const double* eta;      // n
+    const double* bf[3];    // the block's face fields (FACE_IDX), or null
+    double dx[3];
+    double* block_emf[3];   // the block's edge EMFs (EDGE_IDX), or null
+    double* emf;            // n: the value (NaN where status != 0)
+    int* status;            // n
+};
+hipError_t launch_emf_block_batch(const EmfBlockArgs& a, hipStream_t s);
 
 // ---- differentiable scheme (van Albada + HLLE) and its adjoint --------------
 // compact periodic layout [var][X][Y][Z], N = X*Y*Z

@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fx_log.hpp"
+
 namespace fx {
 
 // Largest of the workgroup's values (none of them NaN; blockDim.x a multiple
@@ -425,19 +427,23 @@ __device__ __forceinline__ double emf_upwind(const double* ef, const double* ec,
     return emf;
 }
 
-// Anomalous resistivity mask of one cell (src/prj_timeint.c:230-263).  The
-// log factors are positive only above the thresholds, so they are evaluated
-// only there; the value is identical to max(0, log(.)).
+// Anomalous resistivity mask of one cell (src/prj_timeint.c:230-263):
+// 1e12 * max(0, log(rho / 1e13)) * max(0, log(Ye / 0.25)).  A log factor is
+// positive only when its argument exceeds 1 and is then taken with the
+// correctly rounded log_cr (fx_log.hpp); at or below 1 every faithful log is
+// <= 0 and the factor is 0, so the log is skipped there.  The reference aborts
+// on a non-finite mask; none arises: each factor is at most
+// log(DBL_MAX) < 710, so the mask stays below 1e12 * 710^2.
 __device__ __forceinline__ double eta_mask(double rho, double ye)
 {
     double rf = 0.0, yf = 0.0;
     if (isfinite(rho) && rho > 0.0) {
         double a = rho / 1.0e13;
-        if (a > 1.0) rf = dmax(0.0, log(a));
+        if (a > 1.0) rf = log_cr(a);
     }
     if (isfinite(ye) && ye > 0.0) {
         double b = ye / 0.25;
-        if (b > 1.0) yf = dmax(0.0, log(b));
+        if (b > 1.0) yf = log_cr(b);
     }
     return 1.0e12 * rf * yf;
 }

@@ -308,6 +308,95 @@ __global__ void emf_batch_kernel(int n, const double* ef, const double* ec, cons
     emf[f] = emf_upwind(a, c, v);
 }
 
+// reference block face index (src/prj_defs.h:394-399; 20^3 block, 2 ghosts)
+__device__ __forceinline__ int block_face_idx(int d, int i, int j, int k)
+{
+    constexpr int P = BS + 2 * NG;
+    if (d == 0) return ((i + NG) * P + (j + NG)) * P + (k + NG);
+    if (d == 1) return ((i + NG) * (P + 1) + (j + NG)) * P + (k + NG);
+    return ((i + NG) * P + (j + NG)) * (P + 1) + (k + NG);
+}
+
+// reference block edge index (src/prj_defs.h:400-405)
+__device__ __forceinline__ int block_edge_idx(int d, int i, int j, int k)
+{
+    constexpr int P = BS + 2 * NG;
+    if (d == 0) return ((i + NG) * (P + 1) + (j + NG)) * (P + 1) + (k + NG);
+    if (d == 1) return ((i + NG) * P + (j + NG)) * (P + 1) + (k + NG);
+    return ((i + NG) * (P + 1) + (j + NG)) * P + (k + NG);
+}
+
+__device__ __forceinline__ bool block_face_ok(int d, int i, int j, int k)
+{
+    constexpr int P = BS + 2 * NG;
+    const int c[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (c[a] < -NG || c[a] > (a == d ? P - NG : P - NG - 1)) return false;
+    return true;
+}
+
+// prj_mhd_emf_upwind with the resistive term (src/prj_mhd.c:627-718), one
+// edge of one block per thread.  status: 0, 37 edge outside the block's edge
+// storage (src/prj_mhd.c:636-640), 31 non-finite input, 38 a face array
+// missing for a nonzero eta, 35 a curl read outside the block's face storage
+// (out of bounds in the reference), 36 non-finite curl B or result.
+__global__ void emf_block_batch_kernel(EmfBlockArgs a)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.n) return;
+    const int dir = a.edge[4 * f], i = a.edge[4 * f + 1], j = a.edge[4 * f + 2],
+              k = a.edge[4 * f + 3];
+    int st = 0;
+    double emf = __builtin_nan("");
+    if (dir < 0 || dir > 2 || i < -NG || i >= BS + NG || j < -NG || j >= BS + NG || k < -NG ||
+        k >= BS + NG) {
+        st = 37;
+    } else {
+        double ef[4], ec[4], vn[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ef[q] = a.ef[(int64_t)q * a.n + f];
+            ec[q] = a.ec[(int64_t)q * a.n + f];
+            vn[q] = a.vn[(int64_t)q * a.n + f];
+            if (!isfinite(ef[q]) || !isfinite(ec[q]) || !isfinite(vn[q])) st = 31;
+        }
+        if (st == 0) {
+            double e = emf_upwind(ef, ec, vn);
+            const double eta = a.eta[f];
+            if (eta != 0.0) {
+                // components and shifts of the two differences (src/prj_mhd.c:694-706)
+                const int fa = dir == 0 ? 2 : (dir == 1 ? 0 : 1);
+                const int fb = dir == 0 ? 1 : (dir == 1 ? 2 : 0);
+                const int ia = i - (dir == 2), ja = j - (dir == 0), ka = k - (dir == 1);
+                const int ib = i - (dir == 1), jb = j - (dir == 2), kb = k - (dir == 0);
+                const double wa = a.dx[fa == 2 ? 1 : (fa == 0 ? 2 : 0)];
+                const double wb = a.dx[fb == 1 ? 2 : (fb == 2 ? 0 : 1)];
+                if (a.bf[fa] == nullptr || a.bf[fb] == nullptr) {
+                    st = 38;
+                } else if (!block_face_ok(fa, i, j, k) || !block_face_ok(fa, ia, ja, ka) ||
+                           !block_face_ok(fb, i, j, k) || !block_face_ok(fb, ib, jb, kb)) {
+                    st = 35;
+                } else {
+                    const double curl_b = (a.bf[fa][block_face_idx(fa, i, j, k)] -
+                                           a.bf[fa][block_face_idx(fa, ia, ja, ka)]) / wa -
+                                          (a.bf[fb][block_face_idx(fb, i, j, k)] -
+                                           a.bf[fb][block_face_idx(fb, ib, jb, kb)]) / wb;
+                    if (!isfinite(curl_b)) st = 36;
+                    else e += eta * curl_b;
+                }
+            }
+            if (st == 0 && !isfinite(e)) st = 36;
+            if (st == 0) {
+                emf = e;
+                if (a.block_emf[dir] != nullptr) a.block_emf[dir][block_edge_idx(dir, i, j, k)] = e;
+            }
+        }
+    }
+    a.emf[f] = emf;
+    a.status[f] = st;
+}
+
 }  // namespace
 
 // number of items of the ghost-shell enumeration of fill_shell_cell
@@ -473,6 +562,13 @@ hipError_t launch_hlld_batch(int n, const double* WL, const double* WR, const do
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(hlld_batch_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, WL, WR,
                        pL, pR, gL, gR, bn, flux, vface, bv1, bv2, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_emf_block_batch(const EmfBlockArgs& a, hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(emf_block_batch_kernel, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -134,10 +134,11 @@ def magnetized_sedov(n=(128, 128, 128), xlim=(-1, 1, -1, 1, -1, 1), rho=1.0, p=1
 
 
 def random_smooth(n=(32, 32, 32), seed=0, bc=(PERIODIC,) * 6, gamma=1.4, mhd=False,
-                  vamp=0.5, bamp=0.3, cfl=0.4):
+                  vamp=0.5, bamp=0.3, cfl=0.4, eta=0.0):
     """Smooth random state with O(1) variations, for parity tests.  For MHD the
     face fields come from a random smooth vector potential (so div B = 0 to
-    round-off) plus a uniform field."""
+    round-off) plus a uniform field; `eta` is the uniform CT resistivity
+    (mesh->mhd_eta)."""
     rng = np.random.default_rng(seed)
     xlim = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     nh = 9 if mhd else 6
@@ -176,7 +177,43 @@ def random_smooth(n=(32, 32, 32), seed=0, bc=(PERIODIC,) * 6, gamma=1.4, mhd=Fal
         W[6] = 0.5 * (bf[0][:-1] + bf[0][1:])
         W[7] = 0.5 * (bf[1][:, :-1] + bf[1][:, 1:])
         W[8] = 0.5 * (bf[2][:, :, :-1] + bf[2][:, :, 1:])
-    return Problem("random_smooth", tuple(n), xlim, tuple(bc), gamma, W, bf=bf, mhd=mhd, cfl=cfl)
+    return Problem("random_smooth", tuple(n), xlim, tuple(bc), gamma, W, bf=bf, mhd=mhd, cfl=cfl,
+                   eta=float(eta))
+
+
+# thresholds of the anomalous CT resistivity (src/prj_timeint.c:224-226)
+ETA_RHO_FLOOR = 1.0e13
+ETA_YE_THR = 0.25
+
+
+def dense_resistive(n=(32, 32, 32), seed=0, eta=0.0, ye_amp=1.0e-13, cfl=0.4):
+    """MHD state in the regime of the anomalous CT resistivity: the outflow
+    random_smooth state with density and pressure scaled by 1e13 (rho within
+    a few per cent of the 1e13 floor, above it in about half of the box) and
+    the field by 2^22 (a power of two, so the discrete div B is that of the
+    unscaled state; plasma beta drops by 1.76) -- velocities and sound speeds
+    are those of the unscaled state -- and
+    Ye = 0.25 (1 + ye_amp tanh(g)) for a smooth random g, above the 0.25
+    threshold in about half of the box.  The per-cell mask
+    1e12 log(rho/1e13) log(Ye/0.25) is then positive on the patch where both
+    hold and at most about 1e12 * 0.05 * ye_amp (about 0.005 for the default),
+    which keeps the resistive step well inside explicit stability."""
+    pr = random_smooth(n=n, seed=seed, bc=(OUTFLOW,) * 6, gamma=1.4, mhd=True, eta=eta, cfl=cfl)
+    rng = np.random.default_rng(seed + 7919)
+    W = pr.W
+    W[0] *= 1.0e13             # eint = p / ((gamma - 1) rho) is unchanged
+    W[6:9] *= 2.0 ** 22
+    bf = [b * 2.0 ** 22 for b in pr.bf]
+    g = _lowpass_gaussian(rng, tuple(n), 3)
+    W[5] = ETA_YE_THR * (1.0 + ye_amp * np.tanh(2.0 * g / np.std(g)))
+    return Problem("dense_resistive", tuple(n), pr.xlim, pr.bc, pr.gamma, W, bf=bf, mhd=True,
+                   cfl=cfl, eta=float(eta))
+
+
+def eta_mask_cells(W):
+    """Cells where the anomalous resistivity mask is positive (both log
+    arguments above 1)."""
+    return (W[0] / ETA_RHO_FLOOR > 1.0) & (W[5] / ETA_YE_THR > 1.0)
 
 
 def ragged_ensemble(count=48, sizes=(32, 48, 64, 96, 128), seed=2026, gamma_range=(1.3, 1.7),

@@ -129,10 +129,27 @@ int fx_riemann_hllc_batch(int n, const double *WL, const double *WR, const doubl
 int fx_riemann_hlld_batch(int n, const double *WL, const double *WR, const double *pL,
     const double *pR, const double *gL, const double *gR, const double *bn, double *flux,
     double *v_face, double *Bv1, double *Bv2, int *status, void *stream);
-/* prj_mhd_emf_upwind without the resistive term (src/prj_mhd.h:18-20):
- * emf_face/emf_cell/v_norm 4 components each. */
+/* prj_mhd_emf_upwind without the resistive term (src/prj_mhd.h:18-20, with
+ * eta = 0): emf_face/emf_cell/v_norm 4 components each. */
 int fx_emf_upwind_batch(int n, const double *emf_face, const double *emf_cell,
     const double *v_norm, double *emf, void *stream);
+/* prj_mhd_emf_upwind (src/prj_mhd.h:18-20, src/prj_mhd.c:627-718) with the
+ * resistive term, over n edges of one block in the reference's block layouts.
+ * Item m is the edge (dir, i, j, k) = edge[4m .. 4m+3] (block indices, as the
+ * reference's); emf_face/emf_cell/v_norm 4 components each (device, SoA);
+ * eta[m] the edge's resistivity (mhd_eta plus the mask maximum, as
+ * src/prj_timeint.c:404-405 passes it); bf[3] (a host array of device
+ * pointers) the block's face fields of the stage (FACE_IDX), read only where
+ * eta != 0; dx[3] (host) the block's cell widths.  emf[m] receives the value
+ * and, where block_emf (host array of device pointers) is non-NULL,
+ * block_emf[dir][EDGE_IDX(dir, i, j, k)] too, as the reference stores it.
+ * status[m] = 0, or where the reference aborts: 37 edge outside the block's
+ * storage, 31 non-finite input, 38 missing face field for eta != 0, 36
+ * non-finite curl B or result; 35 a curl read outside the face storage (an
+ * out-of-bounds read in the reference).  emf[m] is NaN where status != 0. */
+int fx_emf_upwind_block_batch(int n, const int *edge, const double *emf_face,
+    const double *emf_cell, const double *v_norm, const double *eta, const double *const *bf,
+    const double *dx, double *const *block_emf, double *emf, int *status, void *stream);
 
 /* ---- per-block entry points in the reference's own block layouts -------- *
  * A block is the reference's 20^3 box (16^3 active cells + 2 ghost layers,

@@ -821,6 +821,85 @@ static double cell_emf(const fxo_grid *g, const double *W, const double *const *
     return e[dir];
 }
 
+/* prj_mhd_emf_upwind with the resistive term on one block's face arrays
+ * (src/prj_mhd.c:627-718): edge (dir, i, j, k) in the reference's block
+ * indices, bf[3] the block's face fields (FACE_IDX, 20^3 block with two ghost
+ * layers), dx the block's cell widths.  Returns 0, 37 (edge index outside
+ * [-2, 17], src/prj_mhd.c:636-640), 31 (non-finite input), 35 (a face read
+ * outside the block's face storage, where the reference reads out of bounds)
+ * or 36 (non-finite curl B or result). */
+#define FXO_BS 20
+#define FXO_FACE_IDX(d, i, j, k) \
+    ((d) == 0 ? (((i) + 2) * FXO_BS * FXO_BS + ((j) + 2) * FXO_BS + ((k) + 2)) : \
+     (d) == 1 ? (((i) + 2) * (FXO_BS + 1) * FXO_BS + ((j) + 2) * FXO_BS + ((k) + 2)) : \
+                (((i) + 2) * FXO_BS * (FXO_BS + 1) + ((j) + 2) * (FXO_BS + 1) + ((k) + 2)))
+
+static int face_in_block(int d, int i, int j, int k)
+{
+    int c[3], a;
+    c[0] = i;
+    c[1] = j;
+    c[2] = k;
+    for (a = 0; a < 3; ++a) {
+        if (c[a] < -2 || c[a] > (a == d ? 18 : 17)) return 0;
+    }
+    return 1;
+}
+
+int fxo_emf_upwind_block(int dir, int i, int j, int k, const double *emf_face,
+    const double *emf_cell, const double *v_norm, const double *const *bf, const double *dx,
+    double eta, double *result)
+{
+    double emf, curl_b;
+    int rc;
+    if (dir < 0 || dir > 2 || i < -2 || i > 17 || j < -2 || j > 17 || k < -2 || k > 17) return 37;
+    rc = fxo_emf_upwind(emf_face, emf_cell, v_norm, &emf);
+    if (rc != 0) return rc;
+    if (eta != 0.0) {
+        int fa, fb;   /* the two face components the curl differences */
+        int ia[3], ib[3], sa[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
+        double wa, wb;
+        if (dir == 0) {
+            fa = 2; fb = 1; sa[1] = 1; sb[2] = 1; wa = dx[1]; wb = dx[2];
+        } else if (dir == 1) {
+            fa = 0; fb = 2; sa[2] = 1; sb[0] = 1; wa = dx[2]; wb = dx[0];
+        } else {
+            fa = 1; fb = 0; sa[0] = 1; sb[1] = 1; wa = dx[0]; wb = dx[1];
+        }
+        ia[0] = i - sa[0]; ia[1] = j - sa[1]; ia[2] = k - sa[2];
+        ib[0] = i - sb[0]; ib[1] = j - sb[1]; ib[2] = k - sb[2];
+        if (!face_in_block(fa, i, j, k) || !face_in_block(fa, ia[0], ia[1], ia[2]) ||
+            !face_in_block(fb, i, j, k) || !face_in_block(fb, ib[0], ib[1], ib[2])) {
+            return 35;
+        }
+        curl_b = (bf[fa][FXO_FACE_IDX(fa, i, j, k)] - bf[fa][FXO_FACE_IDX(fa, ia[0], ia[1], ia[2])]) / wa -
+                 (bf[fb][FXO_FACE_IDX(fb, i, j, k)] - bf[fb][FXO_FACE_IDX(fb, ib[0], ib[1], ib[2])]) / wb;
+        if (!isfinite(curl_b)) return 36;
+        emf += eta * curl_b;
+    }
+    if (!isfinite(emf)) return 36;
+    *result = emf;
+    return 0;
+}
+
+/* log of the resistivity mask: the host libm's, as the reference build
+ * takes it (default), or correctly rounded (fxo_logcr.c) */
+double fxo_log_cr(double x);
+static int g_log_cr = 0;
+
+void fxo_set_log_mode(int correctly_rounded)
+{
+    g_log_cr = correctly_rounded != 0;
+}
+
+static double mask_log(double a)
+{
+    if (!g_log_cr) return log(a);
+    /* correctly rounded log is <= 0 exactly where a <= 1, and max(0, .)
+     * maps that to 0 as it does for the libm value */
+    return a > 1.0 ? fxo_log_cr(a) : 0.0;
+}
+
 /* src/prj_timeint.c:230-263 */
 static double eta_mask(const fxo_grid *g, const double *W, const double *const *bf,
     int x, int y, int z)
@@ -830,10 +909,10 @@ static double eta_mask(const fxo_grid *g, const double *W, const double *const *
     double rf = 0.0;
     double yf = 0.0;
     if (isfinite(rho) && rho > 0.0) {
-        rf = FXO_MAX(0.0, log(rho / 1.0e13));
+        rf = FXO_MAX(0.0, mask_log(rho / 1.0e13));
     }
     if (isfinite(ye) && ye > 0.0) {
-        yf = FXO_MAX(0.0, log(ye / 0.25));
+        yf = FXO_MAX(0.0, mask_log(ye / 0.25));
     }
     return 1.0e12 * rf * yf;
 }

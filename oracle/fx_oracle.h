@@ -65,6 +65,15 @@ int fxo_hlld(const double *WL, const double *WR, double pL, double pR,
     double *Bv1, double *Bv2);
 int fxo_emf_upwind(const double *emf_face, const double *emf_cell,
     const double *v_norm, double *emf);
+/* prj_mhd_emf_upwind with the resistive term on one block's FACE_IDX arrays */
+int fxo_emf_upwind_block(int dir, int i, int j, int k, const double *emf_face,
+    const double *emf_cell, const double *v_norm, const double *const *bf, const double *dx,
+    double eta, double *result);
+
+/* log of the resistivity mask: 0 = the host libm's (as the reference build),
+ * 1 = correctly rounded (as the GPU path) */
+void fxo_set_log_mode(int correctly_rounded);
+double fxo_log_cr(double x);
 
 const char *fxo_last_error(void);
 

@@ -70,6 +70,11 @@ def oracle_lib():
         lib.fxo_hllc.argtypes = [_dp, _dp] + [C.c_double] * 4 + [_dp, _dp] + [C.c_double] * 3
         lib.fxo_hlld.argtypes = [_dp, _dp] + [C.c_double] * 5 + [_dp, _dp, _dp, _dp]
         lib.fxo_emf_upwind.argtypes = [_dp, _dp, _dp, _dp]
+        lib.fxo_set_log_mode.argtypes = [C.c_int]
+        lib.fxo_log_cr.restype = C.c_double
+        lib.fxo_log_cr.argtypes = [C.c_double]
+        lib.fxo_emf_upwind_block.argtypes = [C.c_int] * 4 + [_dp] * 3 + [C.POINTER(_dp), _dp,
+                                                                       C.c_double, _dp]
         lib.fxo_last_error.restype = C.c_char_p
         lib.fxa_va.restype = C.c_double
         lib.fxa_va.argtypes = [C.c_double, C.c_double, C.c_double]
@@ -85,6 +90,30 @@ def oracle_lib():
     return _oracle_lib
 
 
+BLOCK_NFACES = 21 * 20 * 20
+
+
+def oracle_emf_upwind_block(edge, ef, ec, vn, bf, dx, eta):
+    """fxo_emf_upwind_block over items: edge (n, 4) ints, ef/ec/vn (n, 4),
+    bf three FACE_IDX arrays, dx (3,), eta (n,).  Returns (emf, status)."""
+    lib = oracle_lib()
+    n = edge.shape[0]
+    out = np.full(n, np.nan)
+    st = np.zeros(n, dtype=np.int32)
+    bfs = [np.ascontiguousarray(b, dtype=np.float64) for b in bf]
+    bp = (_dp * 3)(*[_ptr(b) for b in bfs])
+    dxa = np.ascontiguousarray(dx, dtype=np.float64)
+    for m in range(n):
+        r = C.c_double(np.nan)
+        a = [np.ascontiguousarray(x[m], dtype=np.float64) for x in (ef, ec, vn)]
+        st[m] = lib.fxo_emf_upwind_block(int(edge[m, 0]), int(edge[m, 1]), int(edge[m, 2]),
+                                         int(edge[m, 3]), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), bp,
+                                         _ptr(dxa), float(eta[m]), C.byref(r))
+        if st[m] == 0:
+            out[m] = r.value
+    return out, st
+
+
 def face_shapes(n):
     nx, ny, nz = n
     return [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
@@ -96,10 +125,13 @@ def edge_shapes(n):
 
 
 class OracleSim:
-    """RK2 stepping with the C restatement on global arrays."""
+    """RK2 stepping with the C restatement on global arrays.  log_cr selects
+    the log of the CT resistivity mask: the host libm's (False, as the
+    reference build) or the correctly rounded one (True, as the GPU path)."""
 
-    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0):
+    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0, log_cr=False):
         self.lib = oracle_lib()
+        self.log_cr = bool(log_cr)
         self.g = _Grid()
         xl = np.ascontiguousarray(xlim, dtype=np.float64)
         bcs = (C.c_int * 6)(*bc)
@@ -143,6 +175,7 @@ class OracleSim:
             eo = (_dp * 3)(*[_ptr(e) for e in emf_out]) if emf_out is not None else nullb
         else:
             b0 = b1 = bo = eo = nullb
+        self.lib.fxo_set_log_mode(int(self.log_cr))
         rc = self.lib.fxo_stage(C.byref(self.g), int(s), float(dt), _ptr(self.W0),
                                 _ptr(self.W1), _ptr(out), b0, b1, bo, eo)
         if rc != 0:
@@ -237,6 +270,8 @@ def ref_lib(mhd, key=None):
             lib.refh_get_emf.argtypes = [_dp, _dp, _dp]
             lib.refh_emf_upwind.restype = C.c_double
             lib.refh_emf_upwind.argtypes = [C.c_int, _dp, _dp, _dp]
+            lib.refh_emf_upwind_block.restype = C.c_double
+            lib.refh_emf_upwind_block.argtypes = [C.c_int] * 4 + [_dp] * 7 + [C.c_double, _dp]
             lib.prj_riemann_hlld.argtypes = [_dp, _dp] + [C.c_double] * 4 + \
                 [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp] + [C.c_double] * 3
         _ref_libs[key] = lib

@@ -407,6 +407,32 @@ double refh_emf_upwind(int dir, const double *emf_face, const double *emf_cell,
     prj_block *b = &g_sim.mesh.blocks[0];
     return prj_mhd_emf_upwind(b, dir, 0, 0, 0, emf_face, emf_cell, v_norm, 0, 0.0);
 }
+
+/* The same with the resistive term: edge (dir, i, j, k) of the scratch block
+ * with its cell widths set to dx, face fields bf[3] (FACE_IDX arrays of
+ * NFACES), resistivity eta.  Returns the value; *stored receives what the
+ * reference wrote to block->emf[dir][EDGE_IDX(dir, i, j, k)]. */
+double refh_emf_upwind_block(int dir, int i, int j, int k, const double *emf_face,
+    const double *emf_cell, const double *v_norm, const double *bfx, const double *bfy,
+    const double *bfz, const double *dx, double eta, double *stored)
+{
+    prj_block *b = &g_sim.mesh.blocks[0];
+    const double *bf[3];
+    double keep[3];
+    double r;
+    int d;
+    bf[0] = bfx;
+    bf[1] = bfy;
+    bf[2] = bfz;
+    for (d = 0; d < 3; ++d) {
+        keep[d] = b->dx[d];
+        b->dx[d] = dx[d];
+    }
+    r = prj_mhd_emf_upwind(b, dir, i, j, k, emf_face, emf_cell, v_norm, bf, eta);
+    *stored = b->emf[dir][EDGE_IDX(dir, i, j, k)];
+    for (d = 0; d < 3; ++d) b->dx[d] = keep[d];
+    return r;
+}
 #endif
 
 /* prj_flux_update (src/prj_flux.c:2149-2521) on raw block arrays.  Inputs in

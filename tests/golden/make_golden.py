@@ -13,7 +13,8 @@ block geometry, levels and neighbour slots) and the SHA-256 of the
 reference's prj_riemann_flux_send / prj_mhd_emf_send outputs on seeded
 block arrays (cf_corrections.json).
 
-    python tests/golden/make_golden.py [--only-block | --only-cf]
+    python tests/golden/make_golden.py [--only-block | --only-cf | --only-emf-block |
+                                        --only-trajectories]
 """
 from __future__ import annotations
 
@@ -126,6 +127,38 @@ def emf_vectors(lib, rng, n=256):
     return dict(emf_face=ef, emf_cell=ec, v_norm=vn, emf=out)
 
 
+def emf_block_vectors(lib, rng, n=384):
+    """prj_mhd_emf_upwind with the resistive term on one block's face arrays:
+    random faces (3 FACE_IDX arrays), edges of every direction over the range
+    the reference's edge loop and its curl reach (block indices -1..16),
+    anisotropic cell widths (not powers of two), eta zero on a quarter of the
+    edges.  `stored` is what the reference wrote to block->emf."""
+    bf = [rng.normal(0.0, 1.0, O.BLOCK_NFACES) for _ in range(3)]
+    dx = np.array([0.03125, 0.05, 0.07])
+    edge = np.zeros((n, 4), dtype=np.int32)
+    edge[:, 0] = rng.integers(0, 3, n)
+    edge[:, 1:] = rng.integers(-1, 17, (n, 3))
+    ef = rng.normal(0.0, 1.0, (n, 4))
+    ec = rng.normal(0.0, 1.0, (n, 4))
+    vn = rng.normal(0.0, 1.0, (n, 4))
+    vn[rng.uniform(size=(n, 4)) < 0.2] = 0.0
+    eta = rng.uniform(0.0, 2.0, n)
+    eta[rng.uniform(size=n) < 0.25] = 0.0
+    out = np.zeros(n)
+    stored = np.zeros(n)
+    bp = [np.ascontiguousarray(b) for b in bf]
+    for m in range(n):
+        a = [np.ascontiguousarray(x[m]) for x in (ef, ec, vn)]
+        st = C.c_double()
+        out[m] = lib.refh_emf_upwind_block(int(edge[m, 0]), int(edge[m, 1]), int(edge[m, 2]),
+                                           int(edge[m, 3]), *[x.ctypes.data_as(_dp) for x in a],
+                                           *[b.ctypes.data_as(_dp) for b in bp],
+                                           dx.ctypes.data_as(_dp), float(eta[m]), C.byref(st))
+        stored[m] = st.value
+    return dict(edge=edge, emf_face=ef, emf_cell=ec, v_norm=vn, eta=eta, bfx=bf[0], bfy=bf[1],
+                bfz=bf[2], dx=dx, emf=out, stored=stored)
+
+
 TRAJECTORIES = {
     # name: (factory, kwargs, steps)
     "sedov_c1_64": ("sedov", dict(n=(64, 64, 64)), 20),
@@ -133,6 +166,10 @@ TRAJECTORIES = {
     "reflect_mixed_32x48x32": ("random_smooth", dict(n=(32, 48, 32), seed=12, bc=(1, 0, 3, 3, 0, 1)), 6),
     "mhd_random_32": ("random_smooth", dict(n=(32, 32, 32), seed=13, bc=(0,) * 6, mhd=True), 6),
     "mhd_sedov_32": ("magnetized_sedov", dict(n=(32, 32, 32)), 6),
+    # resistive CT: uniform mhd_eta, and the anomalous mask on a dense patch
+    "mhd_eta_32": ("random_smooth", dict(n=(32, 32, 32), seed=14, bc=(0,) * 6, mhd=True,
+                                         eta=1.0e-3), 4),
+    "mhd_eta_mask_32": ("dense_resistive", dict(n=(32, 32, 32), seed=3, eta=1.0e-3), 4),
 }
 
 
@@ -229,6 +266,16 @@ def main():
     if "--only-cf" in sys.argv:
         cf_fixtures()
         return
+    if "--only-emf-block" in sys.argv:
+        O.RefSim((32, 32, 32), (0, 1, 0, 1, 0, 1), (0,) * 6, 5.0 / 3.0, mhd=True)
+        np.savez_compressed(os.path.join(HERE, "emf_block_vectors.npz"),
+                            **emf_block_vectors(O.ref_lib(True), np.random.default_rng(20261203)))
+        return
+    if "--only-trajectories" in sys.argv:
+        recs = {name: trajectory(name) for name in TRAJECTORIES}
+        with open(os.path.join(HERE, "trajectories.json"), "w") as f:
+            json.dump(recs, f, indent=1)
+        return
     if "--only-block" in sys.argv:
         with open(os.path.join(HERE, "block_flux.json"), "w") as f:
             json.dump(block_flux(), f, indent=1)
@@ -241,6 +288,8 @@ def main():
     np.savez(os.path.join(HERE, "hllc_vectors.npz"), **hllc_vectors(libh, rng))
     np.savez(os.path.join(HERE, "hlld_vectors.npz"), **hlld_vectors(libm, rng))
     np.savez(os.path.join(HERE, "emf_vectors.npz"), **emf_vectors(libm, rng))
+    np.savez_compressed(os.path.join(HERE, "emf_block_vectors.npz"),
+                        **emf_block_vectors(libm, np.random.default_rng(20261203)))
     recs = {name: trajectory(name) for name in TRAJECTORIES}
     with open(os.path.join(HERE, "trajectories.json"), "w") as f:
         json.dump(recs, f, indent=1)

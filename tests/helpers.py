@@ -42,3 +42,41 @@ def div_b(bf, dx):
     bx, by, bz = bf
     return ((bx[1:] - bx[:-1]) / dx[0] + (by[:, 1:] - by[:, :-1]) / dx[1] +
             (bz[:, :, 1:] - bz[:, :, :-1]) / dx[2])
+
+
+def glibc_log_misrounded(args):
+    """Arguments (float64 > 1) on which the host libm's log -- the reference's
+    and the oracle's -- does not return the correctly rounded double.  The GPU
+    path takes log correctly rounded, so a parity test through log can only be
+    bit-exact where this list is empty.  Screened in long double (64-bit
+    significand), with an exact decimal evaluation for the few arguments whose
+    log lies close to a rounding boundary."""
+    import math
+    from decimal import Decimal, localcontext
+
+    a = np.unique(np.asarray(args, dtype=np.float64).ravel())
+    if a.size == 0:
+        return []
+    g = np.log(a)                                   # numpy's log: screened, not trusted
+    g = np.array([math.log(x) for x in a])          # the host libm's
+    ll = np.log(a.astype(np.longdouble))
+    half = np.spacing(g).astype(np.longdouble) / 2
+    dist = np.abs(ll - g.astype(np.longdouble))
+    margin = np.abs(ll) * np.longdouble(2.0) ** -58
+    bad = list(a[dist > half + margin])
+    for x, gx in zip(a[np.abs(dist - half) <= margin], g[np.abs(dist - half) <= margin]):
+        with localcontext() as c:
+            c.prec = 60
+            cr = float(format(Decimal(float(x)).ln(), ".45e"))
+        if cr != gx:
+            bad.append(float(x))
+    return sorted(set(bad))
+
+
+def eta_mask_log_args(W):
+    """The log arguments of the anomalous-resistivity mask that matter: rho/1e13
+    and Ye/0.25 of the cells where both exceed 1 (elsewhere a factor is 0)."""
+    a = W[0] / 1.0e13
+    b = W[5] / 0.25
+    sel = (a > 1.0) & (b > 1.0)
+    return np.concatenate([a[sel], b[sel]])

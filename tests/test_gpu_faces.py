@@ -84,6 +84,74 @@ def test_emf_batch(gpu):
     assert np.array_equal(out.cpu().numpy(), d["emf"])
 
 
+def _emf_block_call(lib, d, eta=None, edge=None, block_emf=None, bf_present=True):
+    import ctypes as C
+    import torch
+    n = d["edge"].shape[0]
+    edge = d["edge"] if edge is None else edge
+    eta = d["eta"] if eta is None else eta
+    dev_edge = torch.from_numpy(np.ascontiguousarray(edge, dtype=np.int32)).cuda()
+    ins = [_soa(d["emf_face"]), _soa(d["emf_cell"]), _soa(d["v_norm"]), _dev(eta)]
+    bfs = [_dev(d[k]) for k in ("bfx", "bfy", "bfz")]
+    bf_ptrs = (C.c_void_p * 3)(*[b.data_ptr() for b in bfs]) if bf_present else None
+    dx = np.ascontiguousarray(d["dx"], dtype=np.float64)
+    out = torch.zeros(n, dtype=torch.float64, device="cuda")
+    st = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    be = (C.c_void_p * 3)(*[b.data_ptr() for b in block_emf]) if block_emf is not None else None
+    rc = lib.fx_emf_upwind_block_batch(n, dev_edge.data_ptr(), *[a.data_ptr() for a in ins],
+                                       bf_ptrs, dx.ctypes.data_as(C.POINTER(C.c_double)), be,
+                                       out.data_ptr(), st.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), st.cpu().numpy()
+
+
+def test_emf_block_batch_resistive(gpu):
+    """prj_mhd_emf_upwind with eta * curl B on one block's face arrays against
+    the reference's own vectors: returned values and the EDGE_IDX stores."""
+    import torch
+    d = np.load(os.path.join(GOLD, "emf_block_vectors.npz"))
+    nedges = 21 * 21 * 20
+    block_emf = [torch.full((nedges,), np.nan, dtype=torch.float64, device="cuda")
+                 for _ in range(3)]
+    emf, st = _emf_block_call(gpu, d, block_emf=block_emf)
+    assert np.all(st == 0)
+    assert np.array_equal(emf, d["emf"])
+    stores = [b.cpu().numpy() for b in block_emf]
+    P = 20
+    for m, (dr, i, j, k) in enumerate(d["edge"]):
+        if dr == 0:
+            idx = ((i + 2) * (P + 1) + (j + 2)) * (P + 1) + (k + 2)
+        elif dr == 1:
+            idx = ((i + 2) * P + (j + 2)) * (P + 1) + (k + 2)
+        else:
+            idx = ((i + 2) * (P + 1) + (j + 2)) * P + (k + 2)
+        assert stores[dr][idx] == d["stored"][m]
+
+
+def test_emf_block_batch_errors(gpu):
+    """Where the reference aborts, a status per edge: edge outside the block
+    (37), non-finite input (31), no face arrays for eta != 0 (38), a curl read
+    outside the face storage (35)."""
+    d = dict(np.load(os.path.join(GOLD, "emf_block_vectors.npz")))
+    for k in ("edge", "emf_face", "emf_cell", "v_norm", "eta"):
+        d[k] = np.ascontiguousarray(d[k][:6])
+    d["eta"] = np.full(6, 0.5)
+    d["edge"][0] = (0, 18, 3, 3)          # i beyond 17
+    d["emf_face"] = d["emf_face"].copy()
+    d["emf_face"][1, 2] = np.inf
+    d["edge"][2] = (2, -2, 4, 4)          # curl reads face i - 1 = -3
+    d["edge"][3:] = (1, 5, 5, 5)
+    emf, st = _emf_block_call(gpu, d)
+    assert st.tolist() == [37, 31, 35, 0, 0, 0]
+    assert np.isnan(emf[:3]).all() and np.isfinite(emf[3:]).all()
+    _, st = _emf_block_call(gpu, d, bf_present=False)
+    assert st.tolist()[3:] == [38, 38, 38]
+    d["eta"] = np.zeros(6)
+    _, st = _emf_block_call(gpu, d, bf_present=False)
+    assert st.tolist() == [37, 31, 0, 0, 0, 0]
+
+
 def test_empty_batches(gpu):
     """n = 0 on every batched entry point: no launch, FX_OK, outputs untouched;
     a negative count is refused."""
@@ -94,6 +162,7 @@ def test_empty_batches(gpu):
     _native.check(lib.fx_riemann_hllc_batch(0, *([None] * 12)))
     _native.check(lib.fx_riemann_hlld_batch(0, *([None] * 13)))
     _native.check(lib.fx_emf_upwind_batch(0, *([None] * 5)))
+    _native.check(lib.fx_emf_upwind_block_batch(0, *([None] * 11)))
     status = torch.zeros(8, dtype=torch.int32, device="cuda")
     _native.check(lib.fx_flux_update_blocks(0, None, 0, C.c_void_p(status.data_ptr()), None))
     assert lib.fx_riemann_hllc_batch(-1, *([None] * 12)) == -1

@@ -13,11 +13,47 @@ pytestmark = pytest.mark.gpu
 
 
 def _pair(pr):
+    """GPU mesh and oracle on the same state.  The oracle takes the CT
+    resistivity mask's log correctly rounded, as the GPU does (the libm mode
+    is pinned to the reference build in tests/test_oracle_reference.py)."""
     from fornaxamr_amd.mesh import GpuMesh
     gm = GpuMesh.from_problem(pr)
-    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma, mhd=True, eta=pr.eta)
+    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma, mhd=True, eta=pr.eta, log_cr=True)
     om.set_state(pr.W, pr.bf)
     return gm, om
+
+
+RESISTIVE = {
+    # uniform mhd_eta: eta * curl B on every edge (src/prj_mhd.c:689-711)
+    "uniform_eta": lambda: P.random_smooth(n=(32, 32, 32), seed=25, bc=(0,) * 6, mhd=True,
+                                           eta=1.0e-3),
+    # anomalous mask > 0 on a patch with rho > 1e13, Ye > 0.25 (src/prj_timeint.c:230-290)
+    "mask_patch": lambda: P.dense_resistive(n=(32, 32, 32), seed=0),
+    "mask_plus_uniform": lambda: P.dense_resistive(n=(32, 48, 32), seed=1, eta=2.0e-3),
+}
+
+
+@pytest.mark.parametrize("case", sorted(RESISTIVE))
+def test_resistive_ct_steps(gpu, case):
+    """Resistive CT, stage by stage, bit-identical to the oracle: cells, face
+    fields and dt over three RK2 steps."""
+    pr = RESISTIVE[case]()
+    if case != "uniform_eta":
+        assert P.eta_mask_cells(pr.W).sum() > 1000
+    gm, om = _pair(pr)
+    for s in range(3):
+        dt = om.calc_dt(pr.cfl)
+        assert gm.calc_dt(pr.cfl) == dt, s
+        for st in (1, 2):
+            gm.stage(st, dt)
+            om.stage(st, dt)
+            gm.sync()
+            slot = 1 if st == 1 else 0
+            want_W = om.W1 if st == 1 else om.W0
+            want_bf = om.bf1 if st == 1 else om.bf0
+            assert ulp_diff(gm.get_prim(slot), want_W) == 0, (s, st)
+            for a, b in zip(gm.get_bf(slot), want_bf):
+                assert np.array_equal(a, b), (s, st)
 
 
 def test_single_stage(gpu):

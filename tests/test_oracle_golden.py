@@ -61,6 +61,33 @@ def test_emf_vectors():
         assert rc == 0 and out.value == d["emf"][i], i
 
 
+def test_emf_block_vectors():
+    """Resistive per-edge EMF (src/prj_mhd.c:627-718 with eta * curl B) of the
+    oracle against the reference's own prj_mhd_emf_upwind on one block's face
+    arrays: value and the stored edge entry, bit for bit."""
+    d = np.load(os.path.join(GOLD, "emf_block_vectors.npz"))
+    emf, st = O.oracle_emf_upwind_block(d["edge"], d["emf_face"], d["emf_cell"], d["v_norm"],
+                                        [d["bfx"], d["bfy"], d["bfz"]], d["dx"], d["eta"])
+    assert np.all(st == 0)
+    assert np.array_equal(emf, d["emf"])
+    assert np.array_equal(d["stored"], d["emf"])
+    assert np.count_nonzero(d["eta"]) > 0.6 * d["eta"].size
+
+
+def test_emf_block_resistive_term_pins_stencil():
+    """A plausible slip in the curl stencil (shifted index, swapped width) must
+    change the result: the oracle evaluated with the widths permuted differs
+    from the reference vectors on every edge with eta != 0 (so the vectors do
+    pin which face and which width enter)."""
+    d = np.load(os.path.join(GOLD, "emf_block_vectors.npz"))
+    bad_dx = d["dx"][[1, 2, 0]]
+    emf, _ = O.oracle_emf_upwind_block(d["edge"], d["emf_face"], d["emf_cell"], d["v_norm"],
+                                       [d["bfx"], d["bfy"], d["bfz"]], bad_dx, d["eta"])
+    nz = d["eta"] != 0
+    assert np.all(emf[nz] != d["emf"][nz])
+    assert np.array_equal(emf[~nz], d["emf"][~nz])
+
+
 with open(os.path.join(GOLD, "trajectories.json")) as _f:
     TRAJ = json.load(_f)
 

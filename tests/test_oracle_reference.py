@@ -95,3 +95,55 @@ def test_essprk10_4_matches_reference_build(bc):
     for s in range(3):
         assert ref.step(pr.cfl) == om.step_essprk10_4(pr.cfl), s
         assert np.array_equal(ref.get_prim(0), om.W0), s
+
+
+# ---- resistive CT (src/prj_mhd.c:681-711, src/prj_timeint.c:228-290) -------
+
+def _stage_differs(pr, pr0):
+    """One stage of the oracle on pr and on pr0 (same state, no resistivity):
+    the face fields must differ, or a resistive test would pin nothing."""
+    out = []
+    for p in (pr, pr0):
+        o = O.OracleSim(p.n, p.xlim, p.bc, p.gamma, mhd=True, eta=p.eta)
+        o.set_state(p.W, p.bf)
+        o.stage(1, o.calc_dt(p.cfl))
+        out.append([b.copy() for b in o.bf1])
+    return max(float(np.max(np.abs(a - b))) for a, b in zip(*out))
+
+
+def test_mhd_uniform_eta():
+    """mhd_eta = 1e-3 everywhere: eta * curl B on every edge."""
+    pr = P.random_smooth(n=(32, 32, 32), seed=25, bc=(0,) * 6, mhd=True, eta=1.0e-3)
+    pr0 = P.random_smooth(n=(32, 32, 32), seed=25, bc=(0,) * 6, mhd=True)
+    assert _stage_differs(pr, pr0) > 1e-9
+    _run_both(pr, 3)
+
+
+def test_mhd_eta_mask_patch():
+    """Anomalous resistivity: rho > 1e13 and Ye > 0.25 on a patch of the box,
+    so the per-cell mask 1e12 log(rho/1e13) log(Ye/0.25) is positive there."""
+    pr = P.dense_resistive(n=(32, 32, 32), seed=0)
+    assert P.eta_mask_cells(pr.W).sum() > 1000
+    pr0 = P.dense_resistive(n=(32, 32, 32), seed=0)
+    pr0.W[5] = 0.1           # no mask: Ye below the threshold everywhere
+    assert _stage_differs(pr, pr0) > 0.0
+    _run_both(pr, 3)
+
+
+def test_mhd_eta_mask_plus_uniform():
+    _run_both(P.dense_resistive(n=(32, 32, 48), seed=1, eta=2.0e-3), 2)
+
+
+def test_mhd_emf_edges_resistive():
+    pr = P.dense_resistive(n=(32, 32, 32), seed=2, eta=1.0e-3)
+    ref = O.RefSim(pr.n, pr.xlim, pr.bc, pr.gamma, mhd=True, eta=pr.eta)
+    ora = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma, mhd=True, eta=pr.eta)
+    ref.set_state(pr.W, pr.bf)
+    ora.set_state(pr.W, pr.bf)
+    dt = ref.calc_dt(pr.cfl)
+    ref.stage(1, dt)
+    emf = [np.empty(s) for s in O.edge_shapes(pr.n)]
+    ora.stage(1, dt, emf_out=emf)
+    for a, b in zip(ref.get_emf(), emf):
+        assert np.array_equal(a, b)
+    ref.close()
