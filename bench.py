@@ -21,8 +21,9 @@ one set of launches per instance, and the cost of the discrete adjoint (reverse
 sweep with recompute) relative to the forward on the 64^3 acoustic box, and
 the per-block drop-in (fx_flux_update_blocks: prj_flux_update on 4096
 device-resident blocks in the reference's own layouts, one launch).
-``--workload mhd`` prints the line of the MHD stage instead (magnetized Sedov,
-PLM(MC)+HLLD+upwind CT).
+The MHD stage (magnetized Sedov, PLM(MC)+HLLD+upwind CT) at 256^3 and 128^3
+is reported in the same line under ``mhd``; ``--workload mhd`` prints a line
+for the MHD stage alone instead.
 """
 from __future__ import annotations
 
@@ -43,6 +44,11 @@ BYTES_STAGE = {1: 112.0, 2: 160.0}
 # MHD: + B (3 cell, 3 face) read and written, + W^n and Bf^n in stage 2
 BYTES_STAGE_MHD = {1: 208.0, 2: 304.0}
 METRIC = "FP64 zone-updates/s per RK stage (plain vs batched) + adjoint/forward cost ratio"
+
+
+def _progress(msg):
+    """A line on stderr per phase, so a long run shows that it is alive."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def _timed_steps(step, sync, steps):
@@ -137,6 +143,44 @@ BLOCK_FACES = 3 * 17 * 16 * 16
 BYTES_BLOCK = 8.0 * (8 * BLOCK_CELLS + 9 * BLOCK_FACES)
 
 
+def _host_block_call_ms(lib, mhd, reps=50):
+    """Wall time of one fx_flux_update_block call (prj_flux_update on one block
+    in host memory: copies in, one launch, copies out, wait), median of reps."""
+    import ctypes as C
+    import numpy as np
+    import pyoracle as O
+    from fornaxamr_amd import _native
+    slab, gam, bf = O.block_inputs(5, bool(mhd))
+    nh = 9 if mhd else 6
+    eos = O.block_eosvar(slab, 0, gam)
+    W = np.ascontiguousarray(slab.reshape(nh, -1))
+    keep = [W, eos]
+    io = _native.BlockIO()
+    io.W = W.ctypes.data
+    io.var_stride = 2 * BLOCK_CELLS
+    io.eosvar = eos.ctypes.data
+    for d in range(3):
+        for name, size in (("flux", nh * BLOCK_CELLS), ("v_riemann", 3 * BLOCK_CELLS)):
+            a = np.zeros(size)
+            keep.append(a)
+            getattr(io, name)[d] = a.ctypes.data
+        if mhd:
+            b = np.ascontiguousarray(bf[d])
+            keep.append(b)
+            io.bf[d] = b.ctypes.data
+            for name in ("Bv1", "Bv2"):
+                a = np.zeros(BLOCK_CELLS)
+                keep.append(a)
+                getattr(io, name)[d] = a.ctypes.data
+    times = []
+    for r in range(reps + 3):
+        t0 = time.perf_counter()
+        _native.check(lib.fx_flux_update_block(C.byref(io), mhd))
+        if r >= 3:
+            times.append(time.perf_counter() - t0)
+    return float(np.median(times)) * 1000.0
+
+
 def run_per_block(nblocks=4096, reps=10):
     """fx_flux_update_blocks (prj_flux_update over device-resident blocks in the
     reference's own WIDX / EIDX / VIDX / VRIDX layouts), hydro, one launch."""
@@ -191,8 +235,10 @@ def run_per_block(nblocks=4096, reps=10):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
     gbs = BYTES_BLOCK * nblocks / (ms * 1e-3) / 1e9
+    host = {name: _host_block_call_ms(lib, mhd) for name, mhd in (("hydro", 0), ("mhd", 1))}
     return {"kernel": "block_flux_kernel<0> (prj_flux_update, reference block layouts)",
             "blocks": nblocks, "ms_per_launch": ms,
+            "host_array_call_ms": host,
             "faces_per_s": BLOCK_FACES * nblocks / (ms * 1e-3),
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS}}
@@ -217,23 +263,36 @@ def _cpu_worker(args):
     return pr.ncells * 2 * nsteps, dt
 
 
+# CPU share of one GPU on the GPU hosts: worker pools are sized to it
+CPU_SHARE = 16
+
+
 def cpu_baseline(nsteps=6):
-    """Run before any GPU call: the workers are forked from this process."""
+    """Run before any GPU call: the workers are forked from this process.
+    One process per core the process may run on (its affinity set), up to the
+    16-core share of one GPU; each steps the C2 scheme and generator on a
+    64^3 sample box (the reference's per-core rate does not depend on the mesh
+    size: 1.9-2.2e6 zone-updates/s per core at 64^3 and 128^3 alike)."""
     import multiprocessing as mp
     import pyoracle as O
     kind = "reference" if O.ref_available() else "port"
     if kind == "port":
         O.ensure_built()
         os.environ["OMP_NUM_THREADS"] = "1"
-    cores = min(16, os.cpu_count() or 1)
+    affinity = len(os.sched_getaffinity(0))
+    cores = max(1, min(CPU_SHARE, affinity))
     ctx = mp.get_context("fork")
     with ctx.Pool(cores) as pool:
         res = pool.map(_cpu_worker, [(kind, nsteps)] * cores)
     zu = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     return {"value": zu / wall, "unit": "zone-updates/s", "cores": cores, "kind": kind,
-            "sample": (f"{cores} processes x ({nsteps} RK2 steps of the same scheme on a 64^3 "
-                       f"periodic box, 64 blocks of 16^3), one process per core, "
+            "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "per_core": zu / wall / cores,
+            "sample": (f"{cores} processes x ({nsteps} RK2 steps of the C2 scheme, "
+                       f"turbulent_box generator at 64^3 = 64 blocks of 16^3), one process per "
+                       f"core (all {affinity} cores of the affinity set, at most the "
+                       f"{CPU_SHARE}-core share of one GPU), "
                        f"{'serial reference build' if kind == 'reference' else 'C restatement'}")}
 
 
@@ -264,6 +323,21 @@ def run_mesh(pr, bytes_stage, steps, warmup, integrator="rk2"):
     return ncells * stages * steps / elapsed, elapsed / steps * 1000.0, roof
 
 
+def run_mhd_line(n, steps, warmup):
+    """The MHD stage on the magnetized Sedov problem at n^3 (C3 at 128^3; the
+    256^3 line is the headline MHD target): zone-updates/s and the roofline of
+    the stage's kernels (208 / 304 B per zone, event-timed live)."""
+    from fornaxamr_amd import problems as P
+    pr = P.magnetized_sedov(n=(n, n, n))
+    value, ms, roof = run_mesh(pr, BYTES_STAGE_MHD, steps, warmup)
+    roof["kernel"] = "mhd_flux_kernel + mhd_ghostface_kernel + mhd_ct_kernel"
+    roof["traffic"] = _traffic(f"mhd_{n}")
+    return {"value": value, "unit": "zone-updates/s", "ms_per_step": ms,
+            "workload": (f"MHD PLM(MC)+HLLD+upwind CT RK2, {n}^3 magnetized Sedov (outflow), "
+                         f"{pr.nblocks} blocks of 16^3, CFL 0.4"),
+            "roofline": roof}
+
+
 def _traffic(name):
     """HBM bytes per launch from the committed counter summary, if present."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -287,6 +361,8 @@ def main():
                     "default 256 (hydro), 128 (mhd)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble figure")
+    ap.add_argument("--no-mhd", action="store_true",
+                    help="skip the MHD lines (magnetized Sedov at 256^3 and 128^3)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost ratio")
     ap.add_argument("--no-per-block", action="store_true",
                     help="skip the per-block drop-in figure (fx_flux_update_blocks)")
@@ -304,7 +380,9 @@ def main():
     mhd = args.workload == "mhd"
     n = args.n or (128 if mhd else 256)
 
+    _progress("cpu baseline")
     cpu = None if (args.no_cpu or mhd or args.ensemble_only) else cpu_baseline()
+    _progress("gpu lines")
 
     import torch
     from fornaxamr_amd import problems as P
@@ -325,12 +403,10 @@ def main():
         return
 
     if mhd:
+        line = run_mhd_line(n, args.steps, args.warmup)
+        value, ms, roof, workload = (line["value"], line["ms_per_step"], line["roofline"],
+                                     line["workload"])
         pr = P.magnetized_sedov(n=(n, n, n))
-        value, ms, roof = run_mesh(pr, BYTES_STAGE_MHD, args.steps, args.warmup)
-        roof["kernel"] = "mhd_flux_kernel + mhd_ghostface_kernel + mhd_ct_kernel"
-        roof["traffic"] = _traffic(f"mhd_{n}")
-        workload = (f"MHD PLM(MC)+HLLD+upwind CT RK2, {n}^3 magnetized Sedov (outflow), "
-                    f"{pr.nblocks} blocks of 16^3, CFL 0.4")
     else:
         pr = P.turbulent_box(n=(n, n, n), seed=1)
         value, ms, roof = run_mesh(pr, BYTES_STAGE, args.steps, args.warmup, args.integrator)
@@ -357,11 +433,19 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not mhd and not args.no_mhd:
+        out["mhd"] = {}
+        for m in (256, 128):
+            _progress(f"mhd {m}^3")
+            out["mhd"][str(m)] = run_mhd_line(m, args.steps, args.warmup)
     if not mhd and not args.no_ensemble:
+        _progress("ensemble")
         out["batched"] = run_ensemble(args.steps, args.warmup)
     if not mhd and not args.no_adjoint:
+        _progress("adjoint")
         out["adjoint"] = run_adjoint()
     if not mhd and not args.no_per_block:
+        _progress("per-block")
         out["per_block"] = run_per_block()
     print(json.dumps(out), flush=True)
 

@@ -32,6 +32,12 @@ class BlockIO(C.Structure):
                 ("Bv1", C.c_void_p * 3), ("Bv2", C.c_void_p * 3)]
 
 
+class Bf2bcIO(C.Structure):
+    """fx_bf2bc_io: the cell field of one block from its face fields."""
+    _fields_ = [("W", C.c_void_p), ("var_stride", C.c_longlong), ("bf", C.c_void_p * 3),
+                ("done", C.c_void_p)]
+
+
 class CfBlock(C.Structure):
     """fx_cf_block: geometry of one block of a refined mesh and its device arrays."""
     _fields_ = [("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("dx", C.c_double * 3),
@@ -101,6 +107,8 @@ def load():
     lib.fx_adj_backward.argtypes = [vp, vp, vp, vp]
     lib.fx_flux_update_blocks.argtypes = [C.c_int, vp, C.c_int, vp, vp]
     lib.fx_flux_update_block.argtypes = [C.POINTER(BlockIO), C.c_int]
+    lib.fx_bf2bc_blocks.argtypes = [C.c_int, vp, vp, vp]
+    lib.fx_bf2bc_block.argtypes = [C.POINTER(Bf2bcIO)]
     lib.fx_flux_div_block.argtypes = [C.POINTER(_dp), _dp, C.c_double, C.c_int, _dp]
     lib.fx_cf_plan_create.argtypes = [C.c_int, C.POINTER(CfBlock), C.c_int, C.POINTER(CfLink),
                                       C.c_int, C.POINTER(vp)]
@@ -136,6 +144,7 @@ EXPORTS = [
     "fx_ensemble_sync", "fx_ensemble_last_dt", "fx_ensemble_info", "fx_ensemble_autotune",
     "fx_adj_create", "fx_adj_destroy", "fx_adj_forward", "fx_adj_backward",
     "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_div_block",
+    "fx_bf2bc_blocks", "fx_bf2bc_block",
     "fx_mesh_set_integrator",
     "fx_cf_plan_create", "fx_cf_plan_info", "fx_cf_plan_destroy", "fx_cf_flux_send",
     "fx_cf_emf_send",

@@ -200,7 +200,6 @@ __global__ void __launch_bounds__(MHD ? 384 : 256) block_pencil_kernel(
         j = dir == 0 ? pf : (dir == 1 ? n : ps);
         k = dir == 2 ? n : ps;
     };
-    const int sn = dir == 0 ? RB * RB : (dir == 1 ? RB : 1);
     const int sa1 = a1 == 0 ? RB * RB : (a1 == 1 ? RB : 1);
     const int sa2 = a2 == 0 ? RB * RB : (a2 == 1 ? RB : 1);
     int gvar[NHV];
@@ -309,7 +308,45 @@ __global__ void __launch_bounds__(NT) block_flux_div_kernel(const double* __rest
     }
 }
 
+// prj_mhd_bf2bc_impl (src/prj_mhd.c:555-603) over every cell of the 20^3
+// block, ghosts included: cell B = face average (the metric factor is 1 off
+// GR; dividing by 1.0 is the identity).  One workgroup per block.  Cells with
+// a nonzero done flag are skipped (prj_mhd_bf2bc); done == null is
+// prj_mhd_bf2bc_all.  A non-finite average is not stored and is reported in
+// err {1, block, i, j, k} where the reference aborts.
+__global__ void __launch_bounds__(NT) block_bf2bc_kernel(const fx_bf2bc_io* __restrict__ blocks,
+                                                         int* err)
+{
+    const fx_bf2bc_io b = blocks[blockIdx.x];
+    for (int c = threadIdx.x; c < RB * RB * RB; c += NT) {
+        const int k = c % RB - RNG, j = (c / RB) % RB - RNG, i = c / (RB * RB) - RNG;
+        if (b.done != nullptr && b.done[c] != 0) continue;
+        const double b1 = 0.5 * (b.bf[0][face_idx(0, i, j, k)] + b.bf[0][face_idx(0, i + 1, j, k)]);
+        const double b2 = 0.5 * (b.bf[1][face_idx(1, i, j, k)] + b.bf[1][face_idx(1, i, j + 1, k)]);
+        const double b3 = 0.5 * (b.bf[2][face_idx(2, i, j, k)] + b.bf[2][face_idx(2, i, j, k + 1)]);
+        if (!isfinite(b1) || !isfinite(b2) || !isfinite(b3)) {
+            if (err != nullptr && atomicCAS(err, 0, 1) == 0) {
+                err[1] = blockIdx.x;
+                err[2] = i;
+                err[3] = j;
+                err[4] = k;
+            }
+            continue;
+        }
+        b.W[(int64_t)B1 * b.var_stride + c] = b1;
+        b.W[(int64_t)B2 * b.var_stride + c] = b2;
+        b.W[(int64_t)B3 * b.var_stride + c] = b3;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_block_bf2bc(const fx_bf2bc_io* blocks_dev, int n, int* err, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(block_bf2bc_kernel, dim3(n), dim3(NT), 0, s, blocks_dev, err);
+    return hipGetLastError();
+}
 
 hipError_t launch_block_flux(const fx_block_io* blocks_dev, int n, int mhd, int* err,
                              hipStream_t s)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/fornax_gpu.h"
+#include "fx_api.hpp"
 #include "fx_kernels.hpp"
 
 using namespace fx;
@@ -56,7 +57,7 @@ struct fx_mesh {
     std::vector<Span> spans;
 };
 
-namespace {
+namespace fx_api {
 
 thread_local std::string g_err;
 
@@ -70,12 +71,6 @@ int hip_fail(hipError_t e, const char* where)
 {
     return fail(FX_EHIP, std::string(where) + ": " + hipGetErrorString(e));
 }
-
-#define FX_HIP(call)                                    \
-    do {                                                \
-        hipError_t e_ = (call);                         \
-        if (e_ != hipSuccess) return hip_fail(e_, #call); \
-    } while (0)
 
 const char* phys_message(int code)
 {
@@ -96,6 +91,12 @@ const char* phys_message(int code)
     default: return "device error";
     }
 }
+
+}  // namespace fx_api
+
+using namespace fx_api;
+
+namespace {
 
 int check_mesh(fx_mesh* m)
 {
@@ -1150,129 +1151,6 @@ int fx_adj_backward(fx_adj* a, const double* gWT, double* gW0, void* stream)
 
 }  // extern "C"
 
-// ---------------------------------------------------------------------------
-// Per-block entry points on the reference's block layouts.
-namespace {
-
-constexpr size_t RB_CELLS = 8000;    // 20^3
-constexpr size_t RB_FACES = 8400;    // 21 * 20 * 20
-
-// one device allocation carved into the arrays of one block
-struct BlockDev {
-    double* base = nullptr;
-    ~BlockDev()
-    {
-        if (base) (void)hipFree(base);
-    }
-};
-
-}  // namespace
-
-extern "C" {
-
-int fx_flux_update_blocks(int n, const fx_block_io* blocks, int mhd, int* status, void* stream)
-{
-    if (n < 0 || (n > 0 && blocks == nullptr)) return fail(FX_EINVAL, "bad block array");
-    if (mhd && status == nullptr) return fail(FX_EINVAL, "MHD needs a device status word");
-    FX_HIP(launch_block_flux(blocks, n, mhd ? 1 : 0, status, (hipStream_t)stream));
-    return FX_OK;
-}
-
-int fx_flux_update_block(const fx_block_io* hb, int mhd)
-{
-    if (hb == nullptr || hb->W == nullptr || hb->eosvar == nullptr || hb->var_stride < (long long)RB_CELLS)
-        return fail(FX_EINVAL, "bad block descriptor");
-    const int nh = mhd ? 9 : 6;
-    for (int d = 0; d < 3; ++d) {
-        if (hb->flux[d] == nullptr || hb->v_riemann[d] == nullptr)
-            return fail(FX_EINVAL, "missing flux or v_riemann array");
-        if (mhd && (hb->bf[d] == nullptr || hb->Bv1[d] == nullptr || hb->Bv2[d] == nullptr))
-            return fail(FX_EINVAL, "missing MHD face array");
-    }
-    const size_t w_len = (size_t)(nh - 1) * (size_t)hb->var_stride + RB_CELLS;
-    const size_t e_len = 3 * RB_CELLS;
-    const size_t f_len = (size_t)nh * RB_CELLS, v_len = 3 * RB_CELLS;
-    size_t total = w_len + e_len + 3 * (f_len + v_len) + (mhd ? 3 * (RB_FACES + 2 * RB_CELLS) : 0);
-    BlockDev mem;
-    FX_HIP(hipMalloc(&mem.base, total * sizeof(double)));
-    double* p = mem.base;
-    auto take = [&](size_t len) { double* r = p; p += len; return r; };
-    fx_block_io db{};
-    double* dW = take(w_len);
-    double* dE = take(e_len);
-    FX_HIP(hipMemcpy(dW, hb->W, w_len * sizeof(double), hipMemcpyHostToDevice));
-    FX_HIP(hipMemcpy(dE, hb->eosvar, e_len * sizeof(double), hipMemcpyHostToDevice));
-    db.W = dW;
-    db.var_stride = hb->var_stride;
-    db.eosvar = dE;
-    for (int d = 0; d < 3; ++d) {
-        db.flux[d] = take(f_len);
-        db.v_riemann[d] = take(v_len);
-        FX_HIP(hipMemcpy(db.flux[d], hb->flux[d], f_len * sizeof(double), hipMemcpyHostToDevice));
-        FX_HIP(hipMemcpy(db.v_riemann[d], hb->v_riemann[d], v_len * sizeof(double),
-                         hipMemcpyHostToDevice));
-        if (mhd) {
-            double* bfd = take(RB_FACES);
-            FX_HIP(hipMemcpy(bfd, hb->bf[d], RB_FACES * sizeof(double), hipMemcpyHostToDevice));
-            db.bf[d] = bfd;
-            db.Bv1[d] = take(RB_CELLS);
-            db.Bv2[d] = take(RB_CELLS);
-            FX_HIP(hipMemcpy(db.Bv1[d], hb->Bv1[d], RB_CELLS * sizeof(double), hipMemcpyHostToDevice));
-            FX_HIP(hipMemcpy(db.Bv2[d], hb->Bv2[d], RB_CELLS * sizeof(double), hipMemcpyHostToDevice));
-        }
-    }
-    // device copy of the descriptor, then the status word
-    BlockDev aux;
-    FX_HIP(hipMalloc(&aux.base, sizeof(fx_block_io) + 64));
-    fx_block_io* dd = reinterpret_cast<fx_block_io*>(aux.base);
-    int* dstat = reinterpret_cast<int*>(reinterpret_cast<char*>(aux.base) + sizeof(fx_block_io));
-    FX_HIP(hipMemcpy(dd, &db, sizeof(db), hipMemcpyHostToDevice));
-    FX_HIP(hipMemset(dstat, 0, 5 * sizeof(int)));
-    FX_HIP(launch_block_flux(dd, 1, mhd ? 1 : 0, dstat, nullptr));
-    FX_HIP(hipDeviceSynchronize());
-    int st[5];
-    FX_HIP(hipMemcpy(st, dstat, sizeof(st), hipMemcpyDeviceToHost));
-    for (int d = 0; d < 3; ++d) {
-        FX_HIP(hipMemcpy(hb->flux[d], db.flux[d], f_len * sizeof(double), hipMemcpyDeviceToHost));
-        FX_HIP(hipMemcpy(hb->v_riemann[d], db.v_riemann[d], v_len * sizeof(double),
-                         hipMemcpyDeviceToHost));
-        if (mhd) {
-            FX_HIP(hipMemcpy(hb->Bv1[d], db.Bv1[d], RB_CELLS * sizeof(double), hipMemcpyDeviceToHost));
-            FX_HIP(hipMemcpy(hb->Bv2[d], db.Bv2[d], RB_CELLS * sizeof(double), hipMemcpyDeviceToHost));
-        }
-    }
-    if (st[0] != 0) {
-        char buf[256];
-        snprintf(buf, sizeof(buf), "%s (dir %d, face %d %d %d)", phys_message(st[0]), st[1], st[2],
-                 st[3], st[4]);
-        return fail(FX_EPHYS, buf);
-    }
-    return FX_OK;
-}
-
-int fx_flux_div_block(double* const* flux, const double* area, double vol, int nvar,
-                      double* fluxdiv)
-{
-    if (flux == nullptr || area == nullptr || fluxdiv == nullptr || (nvar != 6 && nvar != 9))
-        return fail(FX_EINVAL, "bad flux_div arguments");
-    const size_t f_len = (size_t)nvar * RB_CELLS;
-    BlockDev mem;
-    FX_HIP(hipMalloc(&mem.base, (3 * f_len + (size_t)nvar * 4096) * sizeof(double)));
-    double* f[3];
-    for (int d = 0; d < 3; ++d) {
-        f[d] = mem.base + d * f_len;
-        FX_HIP(hipMemcpy(f[d], flux[d], f_len * sizeof(double), hipMemcpyHostToDevice));
-    }
-    double* out = mem.base + 3 * f_len;
-    FX_HIP(launch_block_flux_div(f[0], f[1], f[2], area, vol, nvar, out, nullptr));
-    FX_HIP(hipDeviceSynchronize());
-    FX_HIP(hipMemcpy(fluxdiv, out, (size_t)nvar * 4096 * sizeof(double), hipMemcpyDeviceToHost));
-    return FX_OK;
-}
-
-// ---- coarse-fine corrections ------------------------------------------------
-
-}  // extern "C"
 
 struct fx_cf_plan {
     int nblocks = 0;

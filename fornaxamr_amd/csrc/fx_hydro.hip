@@ -580,8 +580,12 @@ __device__ __forceinline__ void ens_entry(HydroLds& L, const EnsInst* __restrict
         // dev[2] holds the bits of the instance's largest denominator
         // (non-negative doubles order as their bit patterns); the step clears
         // it before this launch
+        // every thread holds the maximum; a lane of wave 1 raises it, so that
+        // in the persistent kernel this atomic and the next entry's queue
+        // fetch (thread 0, wave 0) never share a wave and the compiler cannot
+        // merge the two single-lane regions across the loop's back edge
         const double m = workgroup_max(d, L.v1p[0]);
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 64)
             atomicMax(reinterpret_cast<unsigned long long*>(I.dev + 2),
                       (unsigned long long)__double_as_longlong(m));
     }
@@ -623,10 +627,12 @@ __global__ void __launch_bounds__(NT) ens_stage_persistent_kernel(const EnsInst*
             __syncthreads();
             if (b >= hi) break;
             ens_entry<STAGE>(L, inst, blocks, b);
-            // Keeps the entry's thread-0 epilogue (the CFL atomic of stage 2)
-            // and the next fetch apart: without it the compiler merges the two
-            // thread-0 regions across the back edge and parks lane 0 outside
-            // the loop while the rest of its wave goes round again (a hang).
+            // The entry's stage-2 epilogue raises the CFL maximum from wave 1
+            // (thread 64), never from the fetching lane; this barrier also
+            // keeps the two apart.  (An earlier build had both on thread 0 and
+            // no barrier here: the compiler merged the two thread-0 regions
+            // across the back edge and parked lane 0 outside the loop while
+            // the rest of its wave went round again -- a hang.)
             __syncthreads();
         }
     }

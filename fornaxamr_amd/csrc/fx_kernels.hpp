@@ -166,6 +166,7 @@ hipError_t launch_adj_step_reverse(const AdjGeom& g, double dt, const double* W0
 // ---- reference block layouts (per-block drop-in) ----------------------------
 hipError_t launch_block_flux(const fx_block_io* blocks_dev, int n, int mhd, int* err,
                              hipStream_t s);
+hipError_t launch_block_bf2bc(const fx_bf2bc_io* blocks_dev, int n, int* err, hipStream_t s);
 hipError_t launch_block_flux_div(const double* f0, const double* f1, const double* f2,
                                  const double* area, double vol, int nvar, double* out,
                                  hipStream_t s);

@@ -16,8 +16,8 @@
 namespace fx {
 
 // Largest of the workgroup's values (none of them NaN; blockDim.x a multiple
-// of 64), valid in thread 0.  lds: one double per wave of LDS that no thread
-// touches until the next barrier after this call.
+// of 64), returned to every thread.  lds: one double per wave of LDS that no
+// thread touches until the next barrier after this call.
 __device__ __forceinline__ double workgroup_max(double v, double* lds)
 {
 #pragma unroll

@@ -44,9 +44,11 @@ class GpuEnsemble:
         _native.check(self.lib.fx_ensemble_create(n, descs, C.byref(h)))
         self.h = h
         self.problems = problems
+        # the state reset() and autotune() return to: the problems' initial
+        # states, replaced per instance by set_state
+        self.states = [None] * n
         for i, p in enumerate(problems):
-            W = np.ascontiguousarray(p.W, dtype=np.float64)
-            _native.check(self.lib.fx_ensemble_set_prim(self.h, i, W.ctypes.data_as(_dp)))
+            self.set_state(i, p.W)
         nb = C.c_int()
         nc = C.c_longlong()
         _native.check(self.lib.fx_ensemble_info(self.h, C.byref(nb), C.byref(nc)))
@@ -65,15 +67,23 @@ class GpuEnsemble:
             pass
 
     def reset(self):
-        """Back to the initial states (both slots) and a fresh dt history."""
-        for i, p in enumerate(self.problems):
-            W = np.ascontiguousarray(p.W, dtype=np.float64)
+        """Back to the reference states (both slots) and a fresh dt history:
+        each instance's initial state, or the last one set_state gave it."""
+        for i, W in enumerate(self.states):
             _native.check(self.lib.fx_ensemble_set_prim(self.h, i, W.ctypes.data_as(_dp)))
 
     def set_state(self, i, W):
-        """State of instance i (both slots) and a fresh dt history for it."""
-        W = np.ascontiguousarray(W, dtype=np.float64)
-        _native.check(self.lib.fx_ensemble_set_prim(self.h, int(i), W.ctypes.data_as(_dp)))
+        """State of instance i (both slots) and a fresh dt history for it; it
+        also becomes the state reset() and autotune() return instance i to."""
+        i = int(i)
+        if not 0 <= i < len(self.problems):
+            raise IndexError(f"instance {i} out of range")
+        W = np.array(W, dtype=np.float64, order="C", copy=True)
+        want = (6,) + tuple(self.problems[i].n)
+        if W.shape != want:
+            raise ValueError(f"instance {i}: W has shape {W.shape}, expected {want}")
+        _native.check(self.lib.fx_ensemble_set_prim(self.h, i, W.ctypes.data_as(_dp)))
+        self.states[i] = W
 
     def set_schedule(self, mode=1, persistent_wgs=0, xcd=1, table=0):
         _native.check(self.lib.fx_ensemble_set_schedule(self.h, int(mode), int(persistent_wgs),
@@ -100,7 +110,7 @@ class GpuEnsemble:
         return W
 
     def autotune(self, cfl=0.4, dt_factor=1.2, reps=3):
-        """Pick the fastest schedule; the state is reset afterwards."""
+        """Pick the fastest schedule; the state is reset afterwards (see reset)."""
         out = (C.c_int * 4)()
         ms = C.c_double()
         _native.check(self.lib.fx_ensemble_autotune(self.h, float(cfl), float(dt_factor), int(reps),

@@ -177,10 +177,34 @@ typedef struct fx_block_io {
  * `stream`, no synchronisation.  MHD: status (device int[5], zeroed by the
  * caller) receives the first HLLD failure {code, dir, i, j, k}. */
 int fx_flux_update_blocks(int n, const fx_block_io *blocks, int mhd, int *status, void *stream);
-/* One prj_flux_update call on host arrays: copies the block in, runs the same
- * kernel with n = 1, copies the outputs back and synchronises; an HLLD failure
- * returns FX_EPHYS with the reference's message. */
+/* One prj_flux_update call on host arrays: copies the block into device memory
+ * the library keeps (allocated on the first call on a device, reused after),
+ * runs the same kernel with n = 1 on a stream of its own, copies the outputs
+ * back and waits for that stream only; an HLLD failure returns FX_EPHYS with
+ * the reference's message.  Calls from several host threads are serialised. */
 int fx_flux_update_block(const fx_block_io *block, int mhd);
+/* prj_mhd_bf2bc / prj_mhd_bf2bc_all (src/prj_mhd.h:7-13, src/prj_mhd.c:555-625):
+ * the cell-centred field W[B1..B3] of every cell of a block, ghosts included,
+ * from the face average of the slot's face fields.  The _mesh variants differ
+ * only in the GR metric factor, which is 1 on this path. */
+typedef struct fx_bf2bc_io {
+    double *W;              /* prims of the stage slot: variable v at W[v*var_stride + LIDX] */
+    long long var_stride;
+    const double *bf[3];    /* the slot's face fields (FACE_IDX) */
+    const int *done;        /* block->cell_derived_done (IDX): flagged cells are skipped
+                             * (prj_mhd_bf2bc); NULL: every cell (prj_mhd_bf2bc_all) */
+} fx_bf2bc_io;
+/* n blocks whose arrays are device memory; `blocks` is a device array of n
+ * descriptors.  One launch on `stream`, no synchronisation.  status (device
+ * int[5], zeroed by the caller, or NULL) receives the first non-finite cell
+ * field {1, block, i, j, k}, where the reference aborts with
+ * "prj_mhd_bf2bc: non-finite cell-centered magnetic field" (that cell is not
+ * written). */
+int fx_bf2bc_blocks(int n, const fx_bf2bc_io *blocks, int *status, void *stream);
+/* One call on host arrays: stages the block through device memory the library
+ * keeps, runs the same kernel with n = 1 on a stream of its own and waits for
+ * it; a non-finite field returns FX_EPHYS with the reference's message. */
+int fx_bf2bc_block(const fx_bf2bc_io *block);
 /* prj_flux_div (src/prj_flux.h:10) at every active cell of one block, on host
  * VIDX flux arrays of nvar variables: fluxdiv[v][i][j][k] (16^3 per v). */
 int fx_flux_div_block(double *const *flux, const double *area, double vol, int nvar,

@@ -471,6 +471,32 @@ def ref_flux_update_raw(mhd, slot, slab, eosvar, bf=None):
     return out
 
 
+def bf2bc_done_flags(seed):
+    """Deterministic cell_derived_done flags of one block (about a third set)."""
+    rng = np.random.default_rng(seed + 1000)
+    return (rng.uniform(size=RB_CELLS) < 0.35).astype(np.int32)
+
+
+def ref_bf2bc_raw(slot, slab, bf, done=None):
+    """prj_mhd_bf2bc (done given: cells with a nonzero flag are skipped) or
+    prj_mhd_bf2bc_all of the serial reference build on one raw MHD block;
+    returns the new W_mhd slab."""
+    lib = ref_lib(True)
+    lib.refh_bf2bc_raw.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, C.c_int]
+    rc = lib.refh_init(1, 1, 1, _ptr(np.array([0.0, 1.0, 0.0, 1.0, 0.0, 1.0])),
+                       (C.c_int * 6)(*([0] * 6)), 1.4, 0.0)
+    if rc:
+        raise RuntimeError(f"refh_init failed ({rc})")
+    out = np.ascontiguousarray(slab, dtype=np.float64).copy()
+    bfa = [np.ascontiguousarray(b, dtype=np.float64) for b in bf]
+    dn = np.ascontiguousarray(done if done is not None else np.zeros(RB_CELLS), dtype=np.int32)
+    rc = lib.refh_bf2bc_raw(int(slot), _ptr(out), *[_ptr(b) for b in bfa],
+                            dn.ctypes.data_as(_ip), int(done is not None))
+    if rc:
+        raise RuntimeError(f"refh_bf2bc_raw failed ({rc})")
+    return out
+
+
 def ref_flux_div_raw(mhd, flux):
     """prj_flux_div at every active cell of a 1-block unit-cube mesh."""
     lib = ref_lib(mhd)

@@ -435,6 +435,39 @@ double refh_emf_upwind_block(int dir, int i, int j, int k, const double *emf_fac
 }
 #endif
 
+/* prj_mhd_bf2bc (use_done != 0, with block->cell_derived_done = done) or
+ * prj_mhd_bf2bc_all (src/prj_mhd.c:555-625) on raw block arrays: the W_mhd
+ * slab and the face fields of `slot` are copied into block 0, the call runs,
+ * and the slab is copied out whole. */
+#if PRJ_MHD
+int refh_bf2bc_raw(int slot, double *W_slab, const double *bfx, const double *bfy,
+    const double *bfz, const int *done, int use_done)
+{
+    prj_block *b;
+    const double *bf[3];
+    int d;
+    if (!g_ready || slot < 0 || slot >= PRJ_BLOCK_NSTAGES) return 1;
+    b = &g_sim.mesh.blocks[0];
+    bf[0] = bfx;
+    bf[1] = bfy;
+    bf[2] = bfz;
+    memcpy(b->W_mhd, W_slab, sizeof(double) * (size_t)PRJ_NHYDRO * PRJ_BLOCK_NSTAGES *
+        PRJ_BLOCK_NCELLS);
+    for (d = 0; d < 3; ++d) {
+        memcpy(prj_block_bf_stage(b, d, slot), bf[d], sizeof(double) * PRJ_BLOCK_NFACES);
+    }
+    if (use_done) {
+        memcpy(b->cell_derived_done, done, sizeof(int) * (size_t)PRJ_BLOCK_NCELLS);
+        prj_mhd_bf2bc(&g_sim.eos, b, slot);
+    } else {
+        prj_mhd_bf2bc_all(&g_sim.eos, b, slot);
+    }
+    memcpy(W_slab, b->W_mhd, sizeof(double) * (size_t)PRJ_NHYDRO * PRJ_BLOCK_NSTAGES *
+        PRJ_BLOCK_NCELLS);
+    return 0;
+}
+#endif
+
 /* prj_flux_update (src/prj_flux.c:2149-2521) on raw block arrays.  Inputs in
  * the reference's own block layouts: W_slab the whole W_mhd slab
  * (NHYDRO x NSTAGES x NCELLS, WIDX), eosvar (EIDX), bf[d] the face fields of

@@ -14,7 +14,7 @@ reference's prj_riemann_flux_send / prj_mhd_emf_send outputs on seeded
 block arrays (cf_corrections.json).
 
     python tests/golden/make_golden.py [--only-block | --only-cf | --only-emf-block |
-                                        --only-trajectories]
+                                        --only-trajectories | --only-bf2bc]
 """
 from __future__ import annotations
 
@@ -222,6 +222,24 @@ def block_flux():
     return recs
 
 
+BF2BC_CASES = {"all_slot0": (31, 0, False), "all_slot1": (32, 1, False),
+               "done_slot1": (33, 1, True)}
+
+
+def bf2bc_fixtures():
+    """prj_mhd_bf2bc(_all) of the reference on raw MHD blocks (inputs from
+    pyoracle.block_inputs(seed, True), the done flags from the seed): SHA-256
+    of the W slab it leaves."""
+    recs = {}
+    for name, (seed, slot, use_done) in BF2BC_CASES.items():
+        slab, _, bf = O.block_inputs(seed, True)
+        done = O.bf2bc_done_flags(seed) if use_done else None
+        out = O.ref_bf2bc_raw(slot, slab, bf, done)
+        recs[name] = dict(seed=seed, slot=slot, use_done=use_done, input_sha=sha(slab),
+                          out_sha=sha(out))
+    return recs
+
+
 CF_MESHES = {
     # two levels, outflow
     "two_level": dict(root=(2, 2, 2), xlim=(-1, 1, -1, 1, -1, 1), bc=(0,) * 6, max_level=2,
@@ -276,6 +294,10 @@ def main():
         with open(os.path.join(HERE, "trajectories.json"), "w") as f:
             json.dump(recs, f, indent=1)
         return
+    if "--only-bf2bc" in sys.argv:
+        with open(os.path.join(HERE, "bf2bc.json"), "w") as f:
+            json.dump(bf2bc_fixtures(), f, indent=1)
+        return
     if "--only-block" in sys.argv:
         with open(os.path.join(HERE, "block_flux.json"), "w") as f:
             json.dump(block_flux(), f, indent=1)
@@ -295,6 +317,8 @@ def main():
         json.dump(recs, f, indent=1)
     with open(os.path.join(HERE, "block_flux.json"), "w") as f:
         json.dump(block_flux(), f, indent=1)
+    with open(os.path.join(HERE, "bf2bc.json"), "w") as f:
+        json.dump(bf2bc_fixtures(), f, indent=1)
     cf_fixtures()
     print("wrote", sorted(os.listdir(HERE)))
 

@@ -127,3 +127,76 @@ def test_bad_block_arguments(gpu):
     assert gpu.fx_flux_update_block(C.byref(io), 0) == -1
     assert gpu.fx_flux_update_blocks(1, None, 0, None, None) == -1
     assert gpu.fx_flux_update_blocks(1, C.c_void_p(8), 1, None, None) == -1
+
+
+# ---- prj_mhd_bf2bc / prj_mhd_bf2bc_all (src/prj_mhd.c:555-625) ----------------
+BF2BC = json.load(open(os.path.join(HERE, "golden", "bf2bc.json")))
+
+
+def _bf2bc_host(lib, slot, slab, bf, done):
+    from fornaxamr_amd import _native
+    out = np.ascontiguousarray(slab, dtype=np.float64).copy()
+    io = _native.Bf2bcIO()
+    io.W = _p(out) + slot * O.RB_CELLS * 8
+    io.var_stride = 2 * O.RB_CELLS
+    bfa = [np.ascontiguousarray(b) for b in bf]
+    for d in range(3):
+        io.bf[d] = _p(bfa[d])
+    dn = np.ascontiguousarray(done, dtype=np.int32) if done is not None else None
+    io.done = _p(dn) if dn is not None else None
+    rc = lib.fx_bf2bc_block(C.byref(io))
+    return rc, out
+
+
+@pytest.mark.parametrize("case", sorted(BF2BC))
+def test_bf2bc_block_matches_reference(gpu, case):
+    g = BF2BC[case]
+    slab, _, bf = O.block_inputs(g["seed"], True)
+    done = O.bf2bc_done_flags(g["seed"]) if g["use_done"] else None
+    rc, out = _bf2bc_host(gpu, g["slot"], slab, bf, done)
+    assert rc == 0
+    assert _sha(out) == g["out_sha"]
+    if O.ref_available():
+        assert np.array_equal(out, O.ref_bf2bc_raw(g["slot"], slab, bf, done))
+
+
+def test_bf2bc_blocks_device_batch(gpu):
+    """fx_bf2bc_blocks over three device-resident blocks in one launch: the
+    same bits as the per-block host calls."""
+    import torch
+    from fornaxamr_amd import _native
+    names = sorted(BF2BC)
+    slabs, bfs, dones, want = [], [], [], []
+    for name in names:
+        g = BF2BC[name]
+        slab, _, bf = O.block_inputs(g["seed"], True)
+        done = O.bf2bc_done_flags(g["seed"]) if g["use_done"] else None
+        slabs.append(torch.from_numpy(slab.copy()).cuda())
+        bfs.append([torch.from_numpy(np.ascontiguousarray(b)).cuda() for b in bf])
+        dones.append(torch.from_numpy(done).cuda() if done is not None else None)
+        want.append(g["out_sha"])
+    descs = (_native.Bf2bcIO * len(names))()
+    for b, name in enumerate(names):
+        io = descs[b]
+        io.W = slabs[b].data_ptr() + BF2BC[name]["slot"] * O.RB_CELLS * 8
+        io.var_stride = 2 * O.RB_CELLS
+        for d in range(3):
+            io.bf[d] = bfs[b][d].data_ptr()
+        io.done = dones[b].data_ptr() if dones[b] is not None else None
+    raw = torch.frombuffer(bytearray(bytes(descs)), dtype=torch.uint8).cuda()
+    status = torch.zeros(8, dtype=torch.int32, device="cuda")
+    _native.check(gpu.fx_bf2bc_blocks(len(names), raw.data_ptr(), status.data_ptr(), None))
+    torch.cuda.synchronize()
+    assert status.cpu().tolist() == [0] * 8
+    for b in range(len(names)):
+        assert _sha(slabs[b].cpu().numpy()) == want[b]
+
+
+def test_bf2bc_nonfinite_field_reports(gpu):
+    """A non-finite face average comes back as the reference's abort message."""
+    slab, _, bf = O.block_inputs(31, True)
+    bf = [b.copy() for b in bf]
+    bf[1][1234] = np.inf
+    rc, _ = _bf2bc_host(gpu, 0, slab, bf, None)
+    assert rc == -3
+    assert b"non-finite cell-centered magnetic field" in gpu.fx_last_error()

@@ -177,3 +177,65 @@ def test_bad_arguments(gpu):
     assert e.lib.fx_ensemble_get_prim(e.h, 1, 0, None) == -1      # FX_EINVAL
     assert e.lib.fx_ensemble_set_schedule(e.h, 1, -1, 0, 0) == -1
     e.close()
+
+
+def test_set_state_checks_shape_and_is_kept_by_reset(gpu):
+    """set_state refuses a state of the wrong shape (it would read past the
+    host array), and reset()/autotune() return an instance to the state
+    set_state gave it, not to the problem's initial one."""
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    probs = _instances()[:2]
+    e = GpuEnsemble(probs)
+    with pytest.raises(ValueError):
+        e.set_state(0, np.zeros((6, 16, 32, 32)))
+    with pytest.raises(ValueError):
+        e.set_state(1, probs[0].W)
+    with pytest.raises(IndexError):
+        e.set_state(2, probs[0].W)
+    W2 = probs[1].W.copy()
+    W2[1:4] *= 0.5
+    e.set_state(1, W2)
+    e.step_async(CFL, FACTOR)
+    e.sync()
+    e.reset()
+    assert np.array_equal(e.get_prim(1, 0), W2)
+    assert np.array_equal(e.get_prim(0, 0), probs[0].W)
+    e.set_graph(True)
+    e.autotune(CFL, FACTOR, reps=1)
+    assert np.array_equal(e.get_prim(1, 0), W2)
+    e.close()
+
+
+def test_full_ragged_ensemble_48(gpu):
+    """C4 at its full size: the 48-member draw (6249 blocks, members up to
+    128^3), autotuned schedule, two graph-replayed steps.  Every instance must
+    be bit-identical (state and dt) to its own single-mesh run, and conserve
+    mass, momentum, energy and rho*Ye in its periodic box to round-off."""
+    from fornaxamr_amd.ensemble import GpuEnsemble
+    from fornaxamr_amd.mesh import GpuMesh
+    from helpers import cons_totals
+    probs = P.ragged_ensemble(48)
+    e = GpuEnsemble(probs)
+    assert e.nblocks == sum(p.nblocks for p in probs)
+    assert max(p.n[0] for p in probs) == 128
+    e.set_graph(True)
+    best = e.autotune(CFL, FACTOR, reps=1)
+    print("autotuned schedule", best, flush=True)
+    dts = []
+    for _ in range(2):
+        e.step_async(CFL, FACTOR)
+        dts.append(e.last_dt())
+    e.sync()
+    for i, pr in enumerate(probs):
+        m = GpuMesh.from_problem(pr)
+        for s in range(2):
+            assert m.step(CFL, FACTOR) == dts[s][i], (i, s)
+        W = e.get_prim(i)
+        assert np.array_equal(W, m.get_prim(0)), i
+        m.close()
+        vol = 1.0 / pr.ncells
+        t0, t1 = cons_totals(pr.W, vol), cons_totals(W, vol)
+        scale = np.abs(t0) + 1e-300
+        scale[1:4] = max(scale[0], float(np.sum(np.abs(pr.W[0] * pr.W[1:4]))) * vol)
+        assert np.all(np.abs(t1 - t0) <= 1e-12 * scale), (i, t1 - t0)
+    e.close()

@@ -131,3 +131,20 @@ def test_block_flux_fixture_matches_reference_build():
         assert {k: [sha(a) for a in v] for k, v in out.items()} == g["out_sha"], name
         fd, _, _ = O.ref_flux_div_raw(g["mhd"], out["flux"])
         assert sha(fd) == g["flux_div_sha"], name
+
+
+def test_bf2bc_fixture_matches_reference_build():
+    """The per-block bf2bc fixture (SHA-256 of the W slab prj_mhd_bf2bc(_all)
+    leaves on seeded raw blocks) reproduces from the serial reference build,
+    and the done flags do skip cells (the two calls differ)."""
+    if not O.ref_available():
+        pytest.skip("serial reference build not present")
+    gold = json.load(open(os.path.join(GOLD, "bf2bc.json")))
+    for name, g in gold.items():
+        slab, _, bf = O.block_inputs(g["seed"], True)
+        assert sha(slab) == g["input_sha"], name
+        done = O.bf2bc_done_flags(g["seed"]) if g["use_done"] else None
+        assert sha(O.ref_bf2bc_raw(g["slot"], slab, bf, done)) == g["out_sha"], name
+    slab, _, bf = O.block_inputs(33, True)
+    assert not np.array_equal(O.ref_bf2bc_raw(1, slab, bf, O.bf2bc_done_flags(33)),
+                              O.ref_bf2bc_raw(1, slab, bf))
