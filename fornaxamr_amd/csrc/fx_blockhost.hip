@@ -28,8 +28,8 @@ constexpr size_t RB_FACES = 8400;    // 21 * 20 * 20 (PRJ_BLOCK_NFACES)
 // Device staging of one block, in doubles: W compact as [var][8000] (at most 9
 // variables), eosvar, flux[3] (up to 9 variables), v_riemann[3], the face
 // fields, Bv1/Bv2[3]; then the descriptor and the status word.
-constexpr size_t ST_W = 9 * RB_CELLS;
-constexpr size_t ST_E = ST_W;
+constexpr size_t ST_W = 0;
+constexpr size_t ST_E = ST_W + 9 * RB_CELLS;
 constexpr size_t ST_F = ST_E + 3 * RB_CELLS;
 constexpr size_t ST_V = ST_F + 3 * 9 * RB_CELLS;
 constexpr size_t ST_BF = ST_V + 3 * 3 * RB_CELLS;

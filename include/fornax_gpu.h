@@ -146,7 +146,9 @@ int fx_emf_upwind_batch(int n, const double *emf_face, const double *emf_cell,
  * status[m] = 0, or where the reference aborts: 37 edge outside the block's
  * storage, 31 non-finite input, 38 missing face field for eta != 0, 36
  * non-finite curl B or result; 35 a curl read outside the face storage (an
- * out-of-bounds read in the reference).  emf[m] is NaN where status != 0. */
+ * out-of-bounds read in the reference).  emf[m] is NaN where status != 0.
+ * Items run concurrently: where one edge appears more than once, its
+ * block_emf entry holds the value of one of them, unspecified which. */
 int fx_emf_upwind_block_batch(int n, const int *edge, const double *emf_face,
     const double *emf_cell, const double *v_norm, const double *eta, const double *const *bf,
     const double *dx, double *const *block_emf, double *emf, int *status, void *stream);
