@@ -107,6 +107,7 @@ def load():
     lib.fx_adj_backward.argtypes = [vp, vp, vp, vp]
     lib.fx_flux_update_blocks.argtypes = [C.c_int, vp, C.c_int, vp, vp]
     lib.fx_flux_update_block.argtypes = [C.POINTER(BlockIO), C.c_int]
+    lib.fx_flux_update_block_noalloc.argtypes = [C.c_int, C.POINTER(BlockIO), C.c_int, vp]
     lib.fx_bf2bc_blocks.argtypes = [C.c_int, vp, vp, vp]
     lib.fx_bf2bc_block.argtypes = [C.POINTER(Bf2bcIO)]
     lib.fx_flux_div_block.argtypes = [C.POINTER(_dp), _dp, C.c_double, C.c_int, _dp]
@@ -143,7 +144,8 @@ EXPORTS = [
     "fx_ensemble_set_schedule", "fx_ensemble_set_graph", "fx_ensemble_step_async",
     "fx_ensemble_sync", "fx_ensemble_last_dt", "fx_ensemble_info", "fx_ensemble_autotune",
     "fx_adj_create", "fx_adj_destroy", "fx_adj_forward", "fx_adj_backward",
-    "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_div_block",
+    "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_update_block_noalloc",
+    "fx_flux_div_block",
     "fx_bf2bc_blocks", "fx_bf2bc_block",
     "fx_mesh_set_integrator",
     "fx_cf_plan_create", "fx_cf_plan_info", "fx_cf_plan_destroy", "fx_cf_flux_send",

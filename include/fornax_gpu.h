@@ -185,6 +185,18 @@ int fx_flux_update_blocks(int n, const fx_block_io *blocks, int mhd, int *status
  * back and waits for that stream only; an HLLD failure returns FX_EPHYS with
  * the reference's message.  Calls from several host threads are serialised. */
 int fx_flux_update_block(const fx_block_io *block, int mhd);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): prj_flux_update on
+ * n host-array blocks, as fx_flux_update_block, without allocating anything or
+ * creating a stream.  The blocks are staged through a fixed device array of
+ * 29,264,128 bytes that the library holds on each GPU it runs on until the
+ * process ends; n outside 1..FX_NOALLOC_MAX_BLOCKS returns FX_EINVAL.  Runs on
+ * `stream` (NULL: the null stream) and waits for it.  Every call on one GPU
+ * shares that array, which keeps the last call's contents (nothing is read
+ * from it that the call has not written); calls from several host threads are
+ * serialised by a lock, as overlapping calls would race on the array. */
+#define FX_NOALLOC_MAX_BLOCKS 8
+int fx_flux_update_block_noalloc(int n, const fx_block_io *blocks, int mhd, void *stream);
 /* prj_mhd_bf2bc / prj_mhd_bf2bc_all (src/prj_mhd.h:7-13, src/prj_mhd.c:555-625):
  * the cell-centred field W[B1..B3] of every cell of a block, ghosts included,
  * from the face average of the slot's face fields.  The _mesh variants differ
