@@ -118,6 +118,7 @@ def load():
     lib.fx_cf_flux_send.argtypes = [vp, vp]
     lib.fx_cf_emf_send.argtypes = [vp, vp, vp]
     lib.fx_mesh_set_integrator.argtypes = [vp, C.c_int]
+    lib.fx_mesh_set_riemann.argtypes = [vp, C.c_int]
     lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
     lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
     lib.fx_emf_upwind_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
@@ -147,7 +148,7 @@ EXPORTS = [
     "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_update_block_noalloc",
     "fx_flux_div_block",
     "fx_bf2bc_blocks", "fx_bf2bc_block",
-    "fx_mesh_set_integrator",
+    "fx_mesh_set_integrator", "fx_mesh_set_riemann",
     "fx_cf_plan_create", "fx_cf_plan_info", "fx_cf_plan_destroy", "fx_cf_flux_send",
     "fx_cf_emf_send",
     "fx_last_error", "fx_version",

@@ -39,6 +39,7 @@ struct fx_mesh {
     bool cfl_ready = false;
     int* err = nullptr;      // [code, dir, x, y, z, stage]
     int integrator = FX_RK2;
+    int riemann = FX_RIEMANN_HLLC;   // hydro meshes
     int ct_split = 1;        // MHD CT workgroups per block
     // MHD meshes whose flux kernel fits the chip in one round (at most 2
     // workgroups per CU): stream and events on which the ghost-layer face
@@ -49,6 +50,7 @@ struct fx_mesh {
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     double* saved[2] = {nullptr, nullptr};   // eSSPRK(10,4) saved states u^n, u5
+    double* saved_bf[2] = {nullptr, nullptr};   // and their face fields (MHD)
     hipStream_t stream = nullptr;
     size_t cell_bytes = 0;   // bytes of one padded variable
     // optional event timing of the stage kernels (excluding the ghost fill)
@@ -180,6 +182,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
         a.dt_ptr = dt_ptr;
         a.dt = dt;
         a.cfl_part = stage == 2 ? m->dev + 8 : nullptr;
+        a.riemann = m->riemann;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (m->timing) {
             FX_HIP(hipEventCreate(&e0));
@@ -345,8 +348,10 @@ int fx_mesh_destroy(fx_mesh* m)
     (void)hipFree(m->U1);
     (void)hipFree(m->fdat);
     (void)hipFree(m->dev);
-    (void)hipFree(m->saved[0]);
-    (void)hipFree(m->saved[1]);
+    for (int s = 0; s < 2; ++s) {
+        (void)hipFree(m->saved[s]);
+        (void)hipFree(m->saved_bf[s]);
+    }
     (void)hipFree(m->err);
     if (m->fork) (void)hipEventDestroy(m->fork);
     if (m->join) (void)hipEventDestroy(m->join);
@@ -432,47 +437,86 @@ int fx_mesh_set_integrator(fx_mesh* m, int integrator)
     if (integrator != FX_RK2 && integrator != FX_ESSPRK10_4)
         return fail(FX_EINVAL, "unknown integrator");
     if (integrator == FX_ESSPRK10_4) {
-        if (m->g.mhd) return fail(FX_EINVAL, "eSSPRK(10,4) is implemented for hydro meshes only");
         FX_HIP(hipStreamSynchronize(m->stream));
         for (int s = 0; s < 2; ++s) {
-            if (m->saved[s] == nullptr) {
-                hipError_t e = hipMalloc(&m->saved[s], m->cell_bytes * m->nh);
-                if (e != hipSuccess)
-                    return fail(e == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,
-                                std::string("fx_mesh_set_integrator: ") + hipGetErrorString(e));
-            }
+            hipError_t e = hipSuccess;
+            if (m->saved[s] == nullptr) e = hipMalloc(&m->saved[s], m->cell_bytes * m->nh);
+            if (e == hipSuccess && m->g.mhd && m->saved_bf[s] == nullptr)
+                e = hipMalloc(&m->saved_bf[s], 3 * m->cell_bytes);
+            if (e != hipSuccess)
+                return fail(e == hipErrorOutOfMemory ? FX_ENOMEM : FX_EHIP,
+                            std::string("fx_mesh_set_integrator: ") + hipGetErrorString(e));
         }
     }
     m->integrator = integrator;
     return FX_OK;
 }
 
+int fx_mesh_set_riemann(fx_mesh* m, int solver)
+{
+    if (int rc = check_mesh(m)) return rc;
+    if (solver != FX_RIEMANN_HLLC && solver != FX_RIEMANN_HLLE)
+        return fail(FX_EINVAL, "unknown Riemann solver");
+    if (m->g.mhd && solver != FX_RIEMANN_HLLC)
+        return fail(FX_EINVAL, "MHD meshes take HLLD fluxes only");
+    FX_HIP(hipStreamSynchronize(m->stream));
+    m->riemann = solver;
+    m->cfl_ready = false;
+    return FX_OK;
+}
+
 namespace {
 
-// One forward-Euler substep of eSSPRK: stage-1 form with the device substep
-// dt, slot 0 -> slot 1, then the slots swap (the state stays in slot 0).
+// One forward-Euler substep of eSSPRK (src/prj_timeint.c:2470-2578): the
+// stage-1 form with the device substep dt, slot 0 -> slot 1 (cells, and the
+// face fields of MHD: the reference's in-place stage-0 CT update of slot 0
+// forms the same values), then the slots swap so that the state stays in
+// slot 0.
 int essprk_substep(fx_mesh* m)
 {
     if (int rc = run_stage(m, 1, m->dev + 4, 0.0)) return rc;
     std::swap(m->W[0], m->W[1]);
+    if (m->g.mhd) std::swap(m->bf[0], m->bf[1]);
+    return FX_OK;
+}
+
+// slot 0 (cells, and faces on MHD meshes) into saved state s
+// (prj_timeint_eSSPRK_save_state, src/prj_timeint.c:2310-2345)
+int essprk_save(fx_mesh* m, int s)
+{
+    FX_HIP(hipMemcpyAsync(m->saved[s], m->W[0], m->cell_bytes * m->nh, hipMemcpyDeviceToDevice,
+                          m->stream));
+    if (m->g.mhd)
+        FX_HIP(hipMemcpyAsync(m->saved_bf[s], m->bf[0], 3 * m->cell_bytes,
+                              hipMemcpyDeviceToDevice, m->stream));
+    return FX_OK;
+}
+
+// slot 0 <- w * saved state s + (1 - w) * slot 0
+// (prj_timeint_eSSPRK_blend_with_saved, src/prj_timeint.c:2382-2417)
+int essprk_blend(fx_mesh* m, int s, double w)
+{
+    if (m->g.mhd)
+        FX_HIP(launch_blend_mhd(m->g, m->W[0], m->saved[s], m->bf[0], m->saved_bf[s], w,
+                                m->stream));
+    else
+        FX_HIP(launch_blend(m->g, m->W[0], m->saved[s], w, m->stream));
     return FX_OK;
 }
 
 // Ketcheson eSSPRK(10,4), low-storage form (src/prj_timeint.c:2661-2730)
 int essprk10_4_step(fx_mesh* m)
 {
-    const size_t bytes = m->cell_bytes * m->nh;
     FX_HIP(launch_div_scalar(m->dev + 0, m->dev + 4, 6.0, m->stream));     // dt / 6
-    FX_HIP(hipMemcpyAsync(m->saved[0], m->W[0], bytes, hipMemcpyDeviceToDevice, m->stream));
+    if (int rc = essprk_save(m, 0)) return rc;
     for (int n = 0; n < 5; ++n)
         if (int rc = essprk_substep(m)) return rc;
-    FX_HIP(hipMemcpyAsync(m->saved[1], m->W[0], bytes, hipMemcpyDeviceToDevice, m->stream));
-    FX_HIP(launch_blend(m->g, m->W[0], m->saved[0], 3.0 / 5.0, m->stream));
+    if (int rc = essprk_save(m, 1)) return rc;
+    if (int rc = essprk_blend(m, 0, 3.0 / 5.0)) return rc;
     for (int n = 0; n < 5; ++n)
         if (int rc = essprk_substep(m)) return rc;
-    FX_HIP(launch_blend(m->g, m->W[0], m->saved[1], 3.0 / 8.0, m->stream));
-    FX_HIP(launch_blend(m->g, m->W[0], m->saved[0], 1.0 / 25.0, m->stream));
-    return FX_OK;
+    if (int rc = essprk_blend(m, 1, 3.0 / 8.0)) return rc;
+    return essprk_blend(m, 0, 1.0 / 25.0);
 }
 
 }  // namespace

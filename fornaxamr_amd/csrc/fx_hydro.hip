@@ -1,6 +1,6 @@
-// Hydro RK2 stage on gfx950: MC reconstruction + HLLC(theta) + flux
-// divergence + per-cell RK2 update, fused in one kernel, one workgroup per
-// 16^3 block.
+// Hydro RK2 stage on gfx950: MC reconstruction + HLLC(theta) (or HLLE) +
+// flux divergence + per-cell RK2 update, fused in one kernel, one workgroup
+// per 16^3 block.
 //
 // Reference path: prj_flux_update (src/prj_flux.c:2149-2521) for every block,
 // then the per-cell stage update (src/prj_timeint.c:1255-1350 stage 1,
@@ -157,12 +157,17 @@ struct HydroLds {
     double hz[NQ][288];      // MC half-slopes along z, cells j=0..15, k=-1..16
 };
 
+// Riemann solver of the hydro stage: the reference's HLLC with the theta
+// limiter, or HLLE (same face states and wave-speed bounds, no contact)
+enum { RS_HLLC = 0, RS_HLLE = 1 };
+
 // One RK2 stage of block (bx, by, bz) of the mesh described by g.  Called by
 // the single-mesh kernel and by the batched ensemble kernels.  Stage 2 returns
 // the largest CFL denominator of the cells this thread wrote (the next step's
 // time-step reduction, taken while the new state is in registers); stage 1
-// returns 0.
-template <int STAGE>
+// returns 0.  RS selects the Riemann solver (the velocity deltas feed HLLC's
+// theta limiter only).
+template <int STAGE, int RS = RS_HLLC>
 __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
                                             const double* __restrict__ Win,
                                             const double* __restrict__ W0,
@@ -280,7 +285,8 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
         double dw = dmin_delta(dmin_delta(A3[c] - A3[c - 1], A3[c + 1] - A3[c]),
                          dmin_delta(B3[c] - B3[c - 1], B3[c + 1] - B3[c]));
         double vf[3];
-        hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fx, vf);
+        if (RS == RS_HLLE) hlle_hydro(WL, WR, pL, pR, gam, gam, Fx, vf);
+        else hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fx, vf);
         __syncthreads();
         load_at(1, r);
         store_plane(pl[1], r, oidx, hidx, has_halo);        // plane 1
@@ -448,7 +454,8 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
                 dw = dir == 1 ? dx1 : din;
             }
             double Fl[6], vf[3];
-            hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fl, vf);
+            if (RS == RS_HLLE) hlle_hydro(WL, WR, pL, pR, gam, gam, Fl, vf);
+            else hllc(WL, WR, pL, pR, gam, gam, du, dv, dw, Fl, vf);
             // back to the global variable order (src/prj_flux.c:609-690)
             double G[6];
             G[0] = Fl[0];
@@ -549,7 +556,7 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
     return cfl_best;
 }
 
-template <int STAGE>
+template <int STAGE, int RS>
 __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
 {
     __shared__ HydroLds L;
@@ -559,7 +566,7 @@ __global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
     const int by = (b / g.nbz) % g.nby;
     const int bx = b / (g.nbz * g.nby);
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt;
-    const double d = hydro_block<STAGE>(L, g, a.Win, a.W0, a.Wout, dt, bx, by, bz);
+    const double d = hydro_block<STAGE, RS>(L, g, a.Win, a.W0, a.Wout, dt, bx, by, bz);
     if (STAGE == 2 && a.cfl_part != nullptr) {
         const double m = workgroup_max(d, L.v1p[0]);
         if (threadIdx.x == 0) a.cfl_part[b] = m;
@@ -648,10 +655,13 @@ __global__ void queue_reset_kernel(int* q)
 hipError_t launch_hydro_stage(const HydroStageArgs& a, int stage, hipStream_t stream)
 {
     const int nblocks = a.g.nbx * a.g.nby * a.g.nbz;
+    const bool hlle = a.riemann == FX_RIEMANN_HLLE;
     if (stage == 1) {
-        hipLaunchKernelGGL(hydro_stage_kernel<1>, dim3(nblocks), dim3(NT), 0, stream, a);
+        if (hlle) hipLaunchKernelGGL((hydro_stage_kernel<1, RS_HLLE>), dim3(nblocks), dim3(NT), 0, stream, a);
+        else hipLaunchKernelGGL((hydro_stage_kernel<1, RS_HLLC>), dim3(nblocks), dim3(NT), 0, stream, a);
     } else {
-        hipLaunchKernelGGL(hydro_stage_kernel<2>, dim3(nblocks), dim3(NT), 0, stream, a);
+        if (hlle) hipLaunchKernelGGL((hydro_stage_kernel<2, RS_HLLE>), dim3(nblocks), dim3(NT), 0, stream, a);
+        else hipLaunchKernelGGL((hydro_stage_kernel<2, RS_HLLC>), dim3(nblocks), dim3(NT), 0, stream, a);
     }
     return hipGetLastError();
 }

@@ -21,6 +21,7 @@ struct HydroStageArgs {
     // stage 2: cfl_part[b] = largest CFL denominator of block b's output
     // (b = (bx * nby + by) * nbz + bz); null: not recorded
     double* cfl_part = nullptr;
+    int riemann = 0;        // FX_RIEMANN_HLLC or FX_RIEMANN_HLLE
 };
 
 struct MhdStageArgs {
@@ -110,6 +111,10 @@ hipError_t launch_dt_final(const double* partial, int n, double cfl, double* dt_
 // eSSPRK convex blend of the active cells: W <- prim(w u(Ws) + (1 - w) u(W))
 hipError_t launch_blend(const Geom& g, double* W, const double* Ws, double saved_weight,
                         hipStream_t s);
+// the same on an MHD mesh: the face fields first (bf <- w bfs + (1 - w) bf),
+// then the cells with their field from the blended faces
+hipError_t launch_blend_mhd(const Geom& g, double* W, const double* Ws, double* bf,
+                            const double* bfs, double saved_weight, hipStream_t s);
 // dst[0] = src[0] / div on the device
 hipError_t launch_div_scalar(const double* src, double* dst, double div, hipStream_t s);
 

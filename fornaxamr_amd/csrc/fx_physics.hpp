@@ -171,6 +171,55 @@ __device__ __forceinline__ void hllc(const double* WL, const double* WR, double 
     flux[5] = Fy + S * (rho_star * y - r * y);
 }
 
+// HLLE: the HLL flux (Harten, Lax & van Leer 1983) over the Davis wave-speed
+// bounds SL = min(uL - cL, uR - cR), SR = max(uL + cL, uR + cR) that the
+// reference's HLLC takes (src/prj_riemann.c:1518-1519), on the reference's
+// face states (prj_riemann_state, src/prj_riemann.c:1433-1492).  The flux and
+// the face velocity (that of the HLL average state) are written in the form of
+// the reference's HLL fallback inside HLLD (src/prj_riemann.c:393-407).  Not
+// a reference solver: the reference has no hydro HLLE.
+__device__ __forceinline__ void hlle_hydro(const double* WL, const double* WR, double pL,
+                                           double pR, double gL, double gR, double* flux,
+                                           double* vface)
+{
+    double rL = WL[RHO], uL = WL[V1], vL = WL[V2], wL = WL[V3], yL = WL[YE];
+    double rR = WR[RHO], uR = WR[V1], vR = WR[V2], wR = WR[V3], yR = WR[YE];
+    double eL = rL * WL[EINT] + 0.5 * rL * (uL * uL + vL * vL + wL * wL);
+    double eR = rR * WR[EINT] + 0.5 * rR * (uR * uR + vR * vR + wR * wR);
+    double csL = sqrt(gL * pL / rL);
+    double csR = sqrt(gR * pR / rR);
+    double UL[6] = {rL, rL * uL, rL * vL, rL * wL, eL, rL * yL};
+    double UR[6] = {rR, rR * uR, rR * vR, rR * wR, eR, rR * yR};
+    double FL[6] = {rL * uL, rL * uL * uL + pL, rL * uL * vL, rL * uL * wL, (eL + pL) * uL,
+                    rL * yL * uL};
+    double FR[6] = {rR * uR, rR * uR * uR + pR, rR * uR * vR, rR * uR * wR, (eR + pR) * uR,
+                    rR * yR * uR};
+    double SL = dmin(uL - csL, uR - csR);
+    double SR = dmax(uL + csL, uR + csR);
+    if (0.0 <= SL) {
+#pragma unroll
+        for (int v = 0; v < 6; ++v) flux[v] = FL[v];
+        vface[0] = uL; vface[1] = vL; vface[2] = wL;
+        return;
+    }
+    if (SR <= 0.0) {
+#pragma unroll
+        for (int v = 0; v < 6; ++v) flux[v] = FR[v];
+        vface[0] = uR; vface[1] = vR; vface[2] = wR;
+        return;
+    }
+    const double inv = 1.0 / (SR - SL);
+    double Uh[4];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+        flux[v] = (SR * FL[v] - SL * FR[v] + SL * SR * (UR[v] - UL[v])) * inv;
+        if (v < 4) Uh[v] = (SR * UR[v] - SL * UL[v] + FL[v] - FR[v]) * inv;
+    }
+    vface[0] = Uh[1] / Uh[0];
+    vface[1] = Uh[2] / Uh[0];
+    vface[2] = Uh[3] / Uh[0];
+}
+
 // ---- HLLD (src/prj_riemann.c:19-440) --------------------------------------
 
 // Error codes mirror the reference abort sites.

@@ -462,6 +462,103 @@ __global__ void __launch_bounds__(256) blend_kernel(Geom g, double* __restrict__
     W[5 * vs + o] = u[5] / rho;
 }
 
+// eSSPRK convex blend of the face fields (src/prj_timeint.c:2347-2371): every
+// active face bf = w_s bf_saved + (1 - w_s) bf.  Items enumerate the three
+// components' active faces, x-faces first.
+__global__ void __launch_bounds__(256) blend_faces_kernel(Geom g, double* __restrict__ bf,
+                                                          const double* __restrict__ bfs,
+                                                          double saved_weight)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nx1 = (int64_t)(g.nx + 1) * g.ny * g.nz, ny1 = (int64_t)g.nx * (g.ny + 1) * g.nz;
+    const int64_t nz1 = (int64_t)g.nx * g.ny * (g.nz + 1);
+    if (id >= nx1 + ny1 + nz1) return;
+    int d;
+    int64_t f;
+    if (id < nx1) { d = 0; f = id; }
+    else if (id < nx1 + ny1) { d = 1; f = id - nx1; }
+    else { d = 2; f = id - nx1 - ny1; }
+    const int ez = g.nz + (d == 2), ey = g.ny + (d == 1);
+    const int z = (int)(f % ez);
+    const int64_t r = f / ez;
+    const int y = (int)(r % ey);
+    const int x = (int)(r / ey);
+    const int64_t o = d * g.fvstride + cell_off(g, x, y, z);
+    bf[o] = saved_weight * bfs[o] + (1.0 - saved_weight) * bf[o];
+}
+
+// eSSPRK convex blend of the active cells of an MHD mesh
+// (src/prj_timeint.c:2382-2417): u = w_s u(W_saved) + (1 - w_s) u(W) with
+// both states' own cell field in their energies, then the cell field from the
+// (already blended) faces, W <- prim(u).
+__global__ void __launch_bounds__(256) blend_mhd_kernel(Geom g, double* __restrict__ W,
+                                                        const double* __restrict__ Ws,
+                                                        const double* __restrict__ bf,
+                                                        double saved_weight)
+{
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const unsigned cid = (unsigned)id;        // cells < 2^31
+    const int z = (int)(cid % (unsigned)g.nz);
+    const unsigned r = cid / (unsigned)g.nz;
+    const int y = (int)(r % (unsigned)g.ny);
+    const int x = (int)(r / (unsigned)g.ny);
+    const int64_t o = cell_off(g, x, y, z);
+    const int64_t vs = g.vstride;
+    const double current_weight = 1.0 - saved_weight;
+    double w[9], ws[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        w[v] = W[v * vs + o];
+        ws[v] = Ws[v * vs + o];
+    }
+    // prim -> cons with field (src/prj_eos.c:2112-2158)
+    double u[6], us[6];
+    u[0] = w[0];
+    u[1] = w[0] * w[1];
+    u[2] = w[0] * w[2];
+    u[3] = w[0] * w[3];
+    u[4] = w[0] * w[4] + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) +
+           0.5 * (w[6] * w[6] + w[7] * w[7] + w[8] * w[8]);
+    u[5] = w[0] * w[5];
+    us[0] = ws[0];
+    us[1] = ws[0] * ws[1];
+    us[2] = ws[0] * ws[2];
+    us[3] = ws[0] * ws[3];
+    us[4] = ws[0] * ws[4] + 0.5 * ws[0] * (ws[1] * ws[1] + ws[2] * ws[2] + ws[3] * ws[3]) +
+            0.5 * (ws[6] * ws[6] + ws[7] * ws[7] + ws[8] * ws[8]);
+    us[5] = ws[0] * ws[5];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) u[v] = saved_weight * us[v] + current_weight * u[v];
+    // cell field from the blended faces (src/prj_timeint.c:498-510)
+    const double* bx = bf;
+    const double* by = bf + g.fvstride;
+    const double* bz = bf + 2 * g.fvstride;
+    const double b1 = 0.5 * (bx[o] + bx[o + g.sx]);
+    const double b2 = 0.5 * (by[o] + by[o + g.sy]);
+    const double b3 = 0.5 * (bz[o] + bz[o + 1]);
+    // cons -> prim with field (src/prj_eos.c:2160-2220)
+    const double rho = u[0];
+    if (rho == 0.0) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) W[v * vs + o] = 0.0;
+        return;
+    }
+    const double v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
+    const double kinetic = 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
+    const double magnetic = 0.5 * (b1 * b1 + b2 * b2 + b3 * b3) / rho;
+    W[o] = rho;
+    W[vs + o] = v1;
+    W[2 * vs + o] = v2;
+    W[3 * vs + o] = v3;
+    W[4 * vs + o] = u[4] / rho - kinetic - magnetic;
+    W[5 * vs + o] = u[5] / rho;
+    W[6 * vs + o] = b1;
+    W[7 * vs + o] = b2;
+    W[8 * vs + o] = b3;
+}
+
 __global__ void scale_kernel(const double* src, double* dst, double div)
 {
     dst[0] = src[0] / div;
@@ -472,6 +569,21 @@ hipError_t launch_blend(const Geom& g, double* W, const double* Ws, double saved
 {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     hipLaunchKernelGGL(blend_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, W, Ws,
+                       saved_weight);
+    return hipGetLastError();
+}
+
+hipError_t launch_blend_mhd(const Geom& g, double* W, const double* Ws, double* bf,
+                            const double* bfs, double saved_weight, hipStream_t s)
+{
+    const int64_t nf = (int64_t)(g.nx + 1) * g.ny * g.nz + (int64_t)g.nx * (g.ny + 1) * g.nz +
+                       (int64_t)g.nx * g.ny * (g.nz + 1);
+    hipLaunchKernelGGL(blend_faces_kernel, dim3(grid_for(nf, 256)), dim3(256), 0, s, g, bf, bfs,
+                       saved_weight);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(blend_mhd_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, W, Ws, bf,
                        saved_weight);
     return hipGetLastError();
 }

@@ -97,8 +97,15 @@ class GpuMesh:
 
     def set_integrator(self, name):
         """Time integrator of step/step_async: "rk2" (default) or "essprk10_4"
-        (Ketcheson's eSSPRK(10,4), the reference's default; hydro only)."""
+        (Ketcheson's eSSPRK(10,4), the reference's default; hydro and MHD)."""
         _native.check(self.lib.fx_mesh_set_integrator(self.h, self.INTEGRATORS[name]))
+
+    RIEMANN = {"hllc": 0, "hlle": 1}
+
+    def set_riemann(self, name):
+        """Riemann solver of a hydro mesh: "hllc" (default, the reference's
+        prj_riemann_hllc) or "hlle" (HLL over the same wave-speed bounds)."""
+        _native.check(self.lib.fx_mesh_set_riemann(self.h, self.RIEMANN[name]))
 
     def step(self, cfl, dt_factor=1.2):
         dt = C.c_double()

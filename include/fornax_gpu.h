@@ -84,10 +84,21 @@ int fx_step(fx_mesh *m, double cfl, double dt_factor, double *dt_used);
  * FX_ESSPRK10_4: Ketcheson's eSSPRK(10,4), the reference's default, as in
  * prj_timeint_eSSPRK10_4_advance (src/prj_timeint.c:2661-2730) -- CFL dt x 6,
  * ten forward-Euler substeps of dt/6 and three convex blends of the conserved
- * state; hydro meshes only.  fx_stage keeps its RK2 meaning. */
+ * state; on MHD meshes the face fields are saved and blended too, before the
+ * cells, whose field is then the face average of the blended faces
+ * (src/prj_timeint.c:2287-2417).  fx_stage keeps its RK2 meaning. */
 #define FX_RK2 0
 #define FX_ESSPRK10_4 1
 int fx_mesh_set_integrator(fx_mesh *m, int integrator);
+/* Riemann solver of a hydro mesh's stage.  FX_RIEMANN_HLLC (default): the
+ * reference's prj_riemann_hllc with the theta limiter (src/prj_riemann.c:
+ * 1494-1599).  FX_RIEMANN_HLLE: HLL over the same Davis wave-speed bounds and
+ * face states, in the form of the reference's HLL fallback
+ * (src/prj_riemann.c:393-407); the reference has no hydro HLLE, so this
+ * option has no reference counterpart.  MHD meshes take HLLD only. */
+#define FX_RIEMANN_HLLC 0
+#define FX_RIEMANN_HLLE 1
+int fx_mesh_set_riemann(fx_mesh *m, int solver);
 /* Waits for the mesh's stream and reports any recorded device error. */
 int fx_sync(fx_mesh *m);
 /* dt of the last device step (synchronises). */

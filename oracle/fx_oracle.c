@@ -680,6 +680,58 @@ static void mc_pair(const double *q, double *wl, double *wr)
     *wr = q[2] - hr;
 }
 
+/* HLLE: the HLL flux (Harten, Lax & van Leer 1983) over the Davis wave-speed
+ * bounds the reference's HLLC takes (src/prj_riemann.c:1518-1519), on its
+ * face states (hllc_state above), written in the form of the reference's HLL
+ * fallback inside HLLD (src/prj_riemann.c:393-407): flux
+ * (SR FL - SL FR + SL SR (UR - UL)) / (SR - SL) and the face velocity of the
+ * HLL average state (SR UR - SL UL + FL - FR) / (SR - SL).  The reference has
+ * no hydro HLLE: this function is pinned by closed forms (the jump conditions
+ * across the two outer waves, consistency, supersonic upwinding, mirror
+ * symmetry) and against the differentiable scheme's independent HLLE. */
+void fxo_hlle(const double *WL, const double *WR, double pL, double pR,
+    double gL, double gR, double *flux, double *v_face)
+{
+    double UL[6], UR[6], FL[6], FR[6], Uh[6];
+    double csL, csR, SL, SR, inv;
+    int v;
+
+    hllc_state(WL, pL, gL, UL, FL, &csL);
+    hllc_state(WR, pR, gR, UR, FR, &csR);
+    SL = FXO_MIN(WL[V1] - csL, WR[V1] - csR);
+    SR = FXO_MAX(WL[V1] + csL, WR[V1] + csR);
+    if (0.0 <= SL) {
+        for (v = 0; v < 6; ++v) flux[v] = FL[v];
+        v_face[0] = WL[V1];
+        v_face[1] = WL[V2];
+        v_face[2] = WL[V3];
+        return;
+    }
+    if (SR <= 0.0) {
+        for (v = 0; v < 6; ++v) flux[v] = FR[v];
+        v_face[0] = WR[V1];
+        v_face[1] = WR[V2];
+        v_face[2] = WR[V3];
+        return;
+    }
+    inv = 1.0 / (SR - SL);
+    for (v = 0; v < 6; ++v) {
+        flux[v] = (SR * FL[v] - SL * FR[v] + SL * SR * (UR[v] - UL[v])) * inv;
+        Uh[v] = (SR * UR[v] - SL * UL[v] + FL[v] - FR[v]) * inv;
+    }
+    v_face[0] = Uh[MOM1] / Uh[RHO];
+    v_face[1] = Uh[MOM2] / Uh[RHO];
+    v_face[2] = Uh[MOM3] / Uh[RHO];
+}
+
+/* Riemann solver of hydro stages: 0 the reference's HLLC, 1 HLLE */
+static int g_riemann = 0;
+
+void fxo_set_riemann(int solver)
+{
+    g_riemann = solver;
+}
+
 static int face_flux(const fxo_grid *g, const double *W, const double *const *bf,
     int dir, int x, int y, int z, face_out *out)
 {
@@ -730,7 +782,11 @@ static int face_flux(const fxo_grid *g, const double *W, const double *const *bf
             FXO_MIN(CV(V1 + a2, x, y, z) - CV(V1 + a2, x - t2[0], y - t2[1], z - t2[2]),
                     CV(V1 + a2, x + t2[0], y + t2[1], z + t2[2]) - CV(V1 + a2, x, y, z)));
 #undef CV
-        fxo_hllc(WL, WR, pL, pR, gL, gR, Fl, vloc, du, dv, dw);
+        if (g_riemann == 1) {
+            fxo_hlle(WL, WR, pL, pR, gL, gR, Fl, vloc);
+        } else {
+            fxo_hllc(WL, WR, pL, pR, gL, gR, Fl, vloc, du, dv, dw);
+        }
         out->bv1 = 0.0;
         out->bv2 = 0.0;
     } else {
@@ -1314,6 +1370,54 @@ double fxo_calc_dt(const fxo_grid *g, const double *W, double cfl)
         }
     }
     return dt_min;
+}
+
+
+/* eSSPRK convex blend with face fields (src/prj_timeint.c:2347-2417, MHD):
+ * first every face bf = w_s bf_saved + (1 - w_s) bf
+ * (prj_timeint_eSSPRK_blend_bf), then per active cell
+ * u = w_s u(W_saved) + (1 - w_s) u(W) with the cell field replaced by the
+ * face average of the blended faces (prj_timeint_mhd_set_cons_b_from_bf),
+ * W <- prim(u) (prj_timeint_eSSPRK_final_cell).                           */
+int fxo_blend_mhd(const fxo_grid *g, double *W, const double *Wsaved, double *const *bf,
+    const double *const *bfsaved, double saved_weight)
+{
+    size_t N = ncell(g);
+    const double current_weight = 1.0 - saved_weight;
+    int d, x;
+    if (!g->mhd) {
+        return fxo_fail(41, "fxo_blend_mhd: MHD only");
+    }
+    for (d = 0; d < 3; ++d) {
+        size_t fcount = (size_t)(g->n[0] + (d == 0)) * (g->n[1] + (d == 1)) * (g->n[2] + (d == 2));
+        size_t f;
+        for (f = 0; f < fcount; ++f) {
+            bf[d][f] = saved_weight * bfsaved[d][f] + current_weight * bf[d][f];
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (x = 0; x < g->n[0]; ++x) {
+        int y, z, v;
+        for (y = 0; y < g->n[1]; ++y) {
+            for (z = 0; z < g->n[2]; ++z) {
+                size_t c = cidx(g, x, y, z);
+                double w[9], ws[9], u[9], us[9];
+                for (v = 0; v < 9; ++v) {
+                    w[v] = W[(size_t)v * N + c];
+                    ws[v] = Wsaved[(size_t)v * N + c];
+                }
+                prim2cons(9, w, u);
+                prim2cons(9, ws, us);
+                for (v = 0; v < 9; ++v) u[v] = saved_weight * us[v] + current_weight * u[v];
+                u[B1] = 0.5 * (bf[0][fidx(g, 0, x, y, z)] + bf[0][fidx(g, 0, x + 1, y, z)]);
+                u[B2] = 0.5 * (bf[1][fidx(g, 1, x, y, z)] + bf[1][fidx(g, 1, x, y + 1, z)]);
+                u[B3] = 0.5 * (bf[2][fidx(g, 2, x, y, z)] + bf[2][fidx(g, 2, x, y, z + 1)]);
+                cons2prim(9, u, w);
+                for (v = 0; v < 9; ++v) W[(size_t)v * N + c] = w[v];
+            }
+        }
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

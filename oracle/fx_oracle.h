@@ -54,12 +54,23 @@ void fxo_bf2bc(const fxo_grid *g, const double *const *bf, double *W);
 /* eSSPRK convex blend of the conserved state (src/prj_timeint.c:2382-2417),
  * hydro: W <- prim(w_s u(W_saved) + (1 - w_s) u(W)) */
 int fxo_blend(const fxo_grid *g, double *W, const double *Wsaved, double saved_weight);
+/* the same with face fields (src/prj_timeint.c:2347-2417, MHD): faces
+ * bf <- w_s bf_saved + (1 - w_s) bf, then cells as above with the cell field
+ * from the blended faces */
+int fxo_blend_mhd(const fxo_grid *g, double *W, const double *Wsaved, double *const *bf,
+    const double *const *bfsaved, double saved_weight);
 
 /* Per-face and per-edge entry points with the reference's argument meaning. */
 double fxo_mc_slope(double qm, double q0, double qp);
 void fxo_hllc(const double *WL, const double *WR, double pL, double pR,
     double gL, double gR, double *flux, double *v_face,
     double du, double dv, double dw);
+/* HLLE over the reference's HLLC wave-speed bounds and face states (not a
+ * reference solver); v_face from the HLL average state */
+void fxo_hlle(const double *WL, const double *WR, double pL, double pR,
+    double gL, double gR, double *flux, double *v_face);
+/* Riemann solver of hydro stages: 0 HLLC (default), 1 HLLE */
+void fxo_set_riemann(int solver);
 int fxo_hlld(const double *WL, const double *WR, double pL, double pR,
     double gL, double gR, double bn, double *flux, double *v_face,
     double *Bv1, double *Bv2);

@@ -65,12 +65,16 @@ def oracle_lib():
                                   C.POINTER(_dp)]
         lib.fxo_bf2bc.argtypes = [C.POINTER(_Grid), C.POINTER(_dp), _dp]
         lib.fxo_blend.argtypes = [C.POINTER(_Grid), _dp, _dp, C.c_double]
+        lib.fxo_blend_mhd.argtypes = [C.POINTER(_Grid), _dp, _dp, C.POINTER(_dp), C.POINTER(_dp),
+                                      C.c_double]
         lib.fxo_mc_slope.restype = C.c_double
         lib.fxo_mc_slope.argtypes = [C.c_double] * 3
         lib.fxo_hllc.argtypes = [_dp, _dp] + [C.c_double] * 4 + [_dp, _dp] + [C.c_double] * 3
         lib.fxo_hlld.argtypes = [_dp, _dp] + [C.c_double] * 5 + [_dp, _dp, _dp, _dp]
         lib.fxo_emf_upwind.argtypes = [_dp, _dp, _dp, _dp]
         lib.fxo_set_log_mode.argtypes = [C.c_int]
+        lib.fxo_hlle.argtypes = [_dp, _dp] + [C.c_double] * 4 + [_dp, _dp]
+        lib.fxo_set_riemann.argtypes = [C.c_int]
         lib.fxo_log_cr.restype = C.c_double
         lib.fxo_log_cr.argtypes = [C.c_double]
         lib.fxo_emf_upwind_block.argtypes = [C.c_int] * 4 + [_dp] * 3 + [C.POINTER(_dp), _dp,
@@ -127,11 +131,15 @@ def edge_shapes(n):
 class OracleSim:
     """RK2 stepping with the C restatement on global arrays.  log_cr selects
     the log of the CT resistivity mask: the host libm's (False, as the
-    reference build) or the correctly rounded one (True, as the GPU path)."""
+    reference build) or the correctly rounded one (True, as the GPU path).
+    riemann: "hllc" (the reference's hydro solver) or "hlle" (hydro only)."""
 
-    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0, log_cr=False):
+    def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0, log_cr=False, riemann="hllc"):
         self.lib = oracle_lib()
         self.log_cr = bool(log_cr)
+        if riemann not in ("hllc", "hlle") or (mhd and riemann != "hllc"):
+            raise ValueError(f"Riemann solver {riemann!r}")
+        self.riemann = 1 if riemann == "hlle" else 0
         self.g = _Grid()
         xl = np.ascontiguousarray(xlim, dtype=np.float64)
         bcs = (C.c_int * 6)(*bc)
@@ -176,6 +184,7 @@ class OracleSim:
         else:
             b0 = b1 = bo = eo = nullb
         self.lib.fxo_set_log_mode(int(self.log_cr))
+        self.lib.fxo_set_riemann(self.riemann)
         rc = self.lib.fxo_stage(C.byref(self.g), int(s), float(dt), _ptr(self.W0),
                                 _ptr(self.W1), _ptr(out), b0, b1, bo, eo)
         if rc != 0:
@@ -189,15 +198,30 @@ class OracleSim:
             if self.mhd:
                 self.bf0 = bfo
 
-    def blend(self, Wsaved, saved_weight):
-        """eSSPRK convex blend of slot 0 with a saved state (hydro)."""
-        rc = self.lib.fxo_blend(C.byref(self.g), _ptr(self.W0), _ptr(Wsaved), float(saved_weight))
+    def blend(self, saved, saved_weight):
+        """eSSPRK convex blend of slot 0 with a saved state: W (hydro), or
+        (W, face fields) (MHD: faces first, then cells with B from them)."""
+        if self.mhd:
+            Ws, bfs = saved
+            b0 = (_dp * 3)(*[_ptr(b) for b in self.bf0])
+            bs = (_dp * 3)(*[_ptr(b) for b in bfs])
+            rc = self.lib.fxo_blend_mhd(C.byref(self.g), _ptr(self.W0), _ptr(Ws), b0, bs,
+                                        float(saved_weight))
+        else:
+            rc = self.lib.fxo_blend(C.byref(self.g), _ptr(self.W0), _ptr(saved),
+                                    float(saved_weight))
         if rc != 0:
             raise RuntimeError(self.lib.fxo_last_error().decode())
 
+    def _save(self):
+        if self.mhd:
+            return self.W0.copy(), [b.copy() for b in self.bf0]
+        return self.W0.copy()
+
     def step_essprk10_4(self, cfl, dt_factor=1.2):
-        """Ketcheson eSSPRK(10,4) step (src/prj_timeint.c:2661-2730, hydro):
-        ten forward-Euler substeps of dt/6 with three convex blends.  The CFL
+        """Ketcheson eSSPRK(10,4) step (src/prj_timeint.c:2661-2730): ten
+        forward-Euler substeps of dt/6 with three convex blends; MHD saves and
+        blends the face fields as well (src/prj_timeint.c:2287-2355).  The CFL
         step is scaled by the SSP coefficient 6 (src/prj_timeint.c:2250-2251)."""
         dt_new = self.calc_dt(cfl) * 6.0
         if self.dt > 0.0:
@@ -211,15 +235,19 @@ class OracleSim:
             for _ in range(5):
                 self.stage(1, dt_sub)
                 self.W0 = self.W1.copy()
+                if self.mhd:
+                    self.bf0 = [b.copy() for b in self.bf1]
 
-        s1 = self.W0.copy()
+        s1 = self._save()
         substeps()
-        s2 = self.W0.copy()
+        s2 = self._save()
         self.blend(s1, 3.0 / 5.0)
         substeps()
         self.blend(s2, 3.0 / 8.0)
         self.blend(s1, 1.0 / 25.0)
         self.W1 = self.W0.copy()
+        if self.mhd:
+            self.bf1 = [b.copy() for b in self.bf0]
         self.time += self.dt
         return self.dt
 
@@ -280,7 +308,9 @@ def ref_lib(mhd, key=None):
 
 class RefSim:
     """The serial reference build driven like src/main.c (one mesh per process).
-    key selects the build: "hydro"/"mhd" (RK2) or "hydro_essprk10_4"."""
+    key selects the build: "hydro"/"mhd" (RK2), "hydro_essprk10_4" /
+    "mhd_essprk10_4" (eSSPRK(10,4)), or "hydro_fxflux"/"mhd_fxflux" (RK2 with
+    prj_flux_update on the GPU through the reference-side shim)."""
 
     def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0, key=None):
         self.lib = ref_lib(mhd, key)
@@ -338,6 +368,17 @@ class RefSim:
 
 
 # ---- differentiable scheme (van Albada + HLLE, fixed dt, periodic) ----------
+
+def hlle(WL, WR, pL, pR, gL, gR):
+    """(flux[6], v_face[3]) of the HLLE face solver on rotated face states."""
+    WL = np.ascontiguousarray(WL, dtype=np.float64)
+    WR = np.ascontiguousarray(WR, dtype=np.float64)
+    F = np.empty(6)
+    vf = np.empty(3)
+    oracle_lib().fxo_hlle(_ptr(WL), _ptr(WR), float(pL), float(pR), float(gL), float(gR),
+                          _ptr(F), _ptr(vf))
+    return F, vf
+
 
 def adj_va(sl, sr, eps2):
     return oracle_lib().fxa_va(float(sl), float(sr), float(eps2))

@@ -79,6 +79,55 @@ def test_sedov_c1_20_steps(gpu):
     assert ulp_diff(gm.get_prim(0), om.W0) == 0
 
 
+def test_sedov_c1_hlle_every_stage(gpu):
+    """C1 as configured: PLM (MC) + HLLE, Sedov 64^3, RK2, CFL 0.4, 20 steps,
+    conserved fields compared after every stage (40 stages) with the oracle's
+    MC + HLLE scheme (fxo_hlle, pinned by closed forms in
+    tests/test_hlle_oracle.py): bit-identical."""
+    from fornaxamr_amd.mesh import GpuMesh
+    pr = P.sedov(n=(64, 64, 64))
+    gm = GpuMesh.from_problem(pr)
+    gm.set_riemann("hlle")
+    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma, riemann="hlle")
+    om.set_state(pr.W)
+    for s in range(20):
+        dt = om.calc_dt(pr.cfl) if s == 0 else min(om.calc_dt(pr.cfl), 1.2 * dt)
+        for st in (1, 2):
+            gm.stage(st, dt)
+            om.stage(st, dt)
+            slot = 1 if st == 1 else 0
+            want = om.W1 if st == 1 else om.W0
+            got = gm.get_prim(slot)
+            assert max_rel(got, want) <= TOL_STAGE, (s, st)
+            assert ulp_diff(got, want) == 0, (s, st)
+    gm.close()
+
+
+@pytest.mark.parametrize("bc", [(3,) * 6, (3, 3, 0, 1, 1, 0)])
+def test_hlle_steps(gpu, bc):
+    """HLLE through the device step (dt on the device, recorded CFL maxima)."""
+    from fornaxamr_amd.mesh import GpuMesh
+    pr = P.random_smooth(n=(32, 48, 32), seed=34, bc=bc, gamma=1.4)
+    gm = GpuMesh.from_problem(pr)
+    gm.set_riemann("hlle")
+    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma, riemann="hlle")
+    om.set_state(pr.W)
+    for s in range(4):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    assert ulp_diff(gm.get_prim(0), om.W0) == 0
+    gm.close()
+
+
+def test_hlle_refused_on_mhd(gpu):
+    from fornaxamr_amd import _native
+    from fornaxamr_amd.mesh import GpuMesh
+    pr = P.random_smooth(n=(32, 32, 32), seed=3, bc=(0,) * 6, mhd=True)
+    gm = GpuMesh.from_problem(pr)
+    with pytest.raises(_native.FxError):
+        gm.set_riemann("hlle")
+    gm.close()
+
+
 def test_periodic_100_steps(gpu):
     """C2 scheme (periodic smooth turbulence) at 64^3 for 100 steps."""
     pr = P.turbulent_box(n=(64, 64, 64), seed=5)

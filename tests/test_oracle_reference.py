@@ -97,6 +97,32 @@ def test_essprk10_4_matches_reference_build(bc):
         assert np.array_equal(ref.get_prim(0), om.W0), s
 
 
+@pytest.mark.parametrize("case", ["random", "sedov", "resistive"])
+def test_essprk10_4_mhd_matches_reference_build(case):
+    """eSSPRK(10,4) on MHD meshes: the face fields are saved with the cells
+    and blended before them, the cell field of a blend taken from the blended
+    faces (src/prj_timeint.c:2287-2355, 2382-2417); oracle against the
+    reference built with PRJ_MHD=1 and TIME_INTEGRATION=eSSPRK10_4."""
+    if not O.ref_available("mhd_essprk10_4"):
+        pytest.skip("serial reference build (MHD eSSPRK10_4) not present")
+    if case == "random":
+        pr = P.random_smooth(n=(32, 32, 32), seed=72, bc=(0,) * 6, mhd=True)
+    elif case == "sedov":
+        pr = P.magnetized_sedov(n=(32, 32, 48))
+    else:
+        pr = P.dense_resistive(n=(32, 32, 32), seed=73, eta=1.0e-3)
+    ref = O.RefSim(pr.n, pr.xlim, pr.bc, pr.gamma, mhd=True, eta=pr.eta, key="mhd_essprk10_4")
+    ref.set_state(pr.W, pr.bf)
+    om = O.OracleSim(pr.n, pr.xlim, pr.bc, pr.gamma, mhd=True, eta=pr.eta)
+    om.set_state(pr.W, pr.bf)
+    for s in range(2):
+        assert ref.step(pr.cfl) == om.step_essprk10_4(pr.cfl), s
+        assert np.array_equal(ref.get_prim(0), om.W0), s
+        for a, b in zip(ref.get_bf(0), om.bf0):
+            assert np.array_equal(a, b), s
+    ref.close()
+
+
 # ---- resistive CT (src/prj_mhd.c:681-711, src/prj_timeint.c:228-290) -------
 
 def _stage_differs(pr, pr0):
