@@ -119,6 +119,8 @@ def load():
     lib.fx_cf_emf_send.argtypes = [vp, vp, vp]
     lib.fx_mesh_set_integrator.argtypes = [vp, C.c_int]
     lib.fx_mesh_set_riemann.argtypes = [vp, C.c_int]
+    lib.fx_mesh_set_mhd_schedule.argtypes = [vp, C.c_int, C.c_int]
+    lib.fx_reconstruct_mc_slope_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
     lib.fx_riemann_hllc_batch.argtypes = [C.c_int] + [vp] * 11 + [vp]
     lib.fx_riemann_hlld_batch.argtypes = [C.c_int] + [vp] * 12 + [vp]
     lib.fx_emf_upwind_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp]
@@ -139,7 +141,8 @@ EXPORTS = [
     "fx_mesh_get_prim", "fx_mesh_set_bf", "fx_mesh_get_bf", "fx_calc_dt", "fx_stage",
     "fx_step_async", "fx_step", "fx_sync", "fx_last_dt", "fx_mesh_view_get",
     "fx_mesh_set_timing", "fx_mesh_kernel_time",
-    "fx_riemann_hllc_batch", "fx_riemann_hlld_batch", "fx_emf_upwind_batch",
+    "fx_reconstruct_mc_slope_batch", "fx_riemann_hllc_batch", "fx_riemann_hlld_batch",
+    "fx_emf_upwind_batch",
     "fx_emf_upwind_block_batch",
     "fx_ensemble_create", "fx_ensemble_destroy", "fx_ensemble_set_prim", "fx_ensemble_get_prim",
     "fx_ensemble_set_schedule", "fx_ensemble_set_graph", "fx_ensemble_step_async",
@@ -148,7 +151,7 @@ EXPORTS = [
     "fx_flux_update_blocks", "fx_flux_update_block", "fx_flux_update_block_noalloc",
     "fx_flux_div_block",
     "fx_bf2bc_blocks", "fx_bf2bc_block",
-    "fx_mesh_set_integrator", "fx_mesh_set_riemann",
+    "fx_mesh_set_integrator", "fx_mesh_set_riemann", "fx_mesh_set_mhd_schedule",
     "fx_cf_plan_create", "fx_cf_plan_info", "fx_cf_plan_destroy", "fx_cf_flux_send",
     "fx_cf_emf_send",
     "fx_last_error", "fx_version",

@@ -49,6 +49,7 @@ struct fx_mesh {
     // 2.7 % faster at 128^3).
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    bool side_on = false;    // the ghost-layer face kernel runs on `side`
     double* saved[2] = {nullptr, nullptr};   // eSSPRK(10,4) saved states u^n, u5
     double* saved_bf[2] = {nullptr, nullptr};   // and their face fields (MHD)
     hipStream_t stream = nullptr;
@@ -167,6 +168,15 @@ int copy_faces(fx_mesh* m, double* dev, double* const* host_in, double* const* h
     return FX_OK;
 }
 
+// the second stream and the fork/join events of the MHD ghost-layer face kernel
+hipError_t create_side_stream(fx_mesh* m)
+{
+    hipError_t e = hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->join, hipEventDisableTiming);
+    return e;
+}
+
 int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
 {
     const Geom& g = m->g;
@@ -224,7 +234,8 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
         FX_HIP(hipEventCreate(&e1));
         FX_HIP(hipEventRecord(e0, m->stream));
     }
-    FX_HIP(launch_mhd_stage(a, stage, m->stream, m->side, m->fork, m->join));
+    FX_HIP(launch_mhd_stage(a, stage, m->stream, m->side_on ? m->side : nullptr, m->fork,
+                            m->join));
     m->cfl_ready = stage == 2;
     if (m->timing) {
         FX_HIP(hipEventRecord(e1, m->stream));
@@ -304,16 +315,16 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
         if (e == hipSuccess) m->ct_split = mhd_ct_split(m->g, ncu);
         const int64_t nb = (int64_t)m->g.nbx * m->g.nby * m->g.nbz;
         if (e == hipSuccess && nb <= 2 * (int64_t)ncu) {
-            e = hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&m->join, hipEventDisableTiming);
+            e = create_side_stream(m);
+            m->side_on = e == hipSuccess;
         }
     }
     // dev[0..7] scalars, then the CFL partials: one per workgroup of the
     // reduction, or the maxima a stage 2 records
     m->nblocks = m->g.nbx * m->g.nby * m->g.nbz;
     m->cfl_nparts = m->nblocks * (desc->mhd ? m->ct_split : 1);
-    const size_t npart = (size_t)std::max(CALC_DT_WGS, m->cfl_nparts);
+    // room for the partials of the largest CT split fx_mesh_set_mhd_schedule allows
+    const size_t npart = (size_t)std::max(CALC_DT_WGS, m->nblocks * (desc->mhd ? 4 : 1));
     if (e == hipSuccess) e = hipMalloc(&m->dev, (8 + npart) * sizeof(double));
     if (e == hipSuccess) e = hipMemset(m->dev, 0, 8 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&m->err, 8 * sizeof(int));
@@ -449,6 +460,28 @@ int fx_mesh_set_integrator(fx_mesh* m, int integrator)
         }
     }
     m->integrator = integrator;
+    return FX_OK;
+}
+
+int fx_mesh_set_mhd_schedule(fx_mesh* m, int ct_split, int side_stream)
+{
+    if (int rc = check_mesh(m)) return rc;
+    if (!m->g.mhd) return fail(FX_EINVAL, "fx_mesh_set_mhd_schedule: not an MHD mesh");
+    if (ct_split != 0 && ct_split != 1 && ct_split != 2 && ct_split != 4)
+        return fail(FX_EINVAL, "ct_split must be 0 (automatic), 1, 2 or 4");
+    if (side_stream < -1 || side_stream > 1)
+        return fail(FX_EINVAL, "side_stream must be -1 (automatic), 0 or 1");
+    FX_HIP(hipStreamSynchronize(m->stream));
+    if (m->side) FX_HIP(hipStreamSynchronize(m->side));
+    int dev_id = 0, ncu = 0;
+    FX_HIP(hipGetDevice(&dev_id));
+    FX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_id));
+    m->ct_split = ct_split != 0 ? ct_split : mhd_ct_split(m->g, ncu);
+    m->cfl_nparts = m->nblocks * m->ct_split;
+    m->cfl_ready = false;   // recorded per CT workgroup of the old split
+    bool on = side_stream == 1 || (side_stream == -1 && m->nblocks <= 2 * ncu);
+    if (on && m->side == nullptr) FX_HIP(create_side_stream(m));
+    m->side_on = on;
     return FX_OK;
 }
 
@@ -607,6 +640,15 @@ int fx_mesh_view_get(fx_mesh* m, fx_mesh_view* out)
     out->n[1] = m->g.ny;
     out->n[2] = m->g.nz;
     out->nh = m->nh;
+    return FX_OK;
+}
+
+int fx_reconstruct_mc_slope_batch(int n, const double* qm, const double* q0, const double* qp,
+                                  double* slope, void* stream)
+{
+    if (n < 0 || (n > 0 && (qm == nullptr || q0 == nullptr || qp == nullptr || slope == nullptr)))
+        return fail(FX_EINVAL, "bad mc slope arguments");
+    FX_HIP(launch_mc_slope_batch(n, qm, q0, qp, slope, (hipStream_t)stream));
     return FX_OK;
 }
 

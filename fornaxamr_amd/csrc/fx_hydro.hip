@@ -371,6 +371,7 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
         const bool extra = wave == xw && lane < 32;
         __builtin_amdgcn_s_setprio(0);
         if (wv == xw) __builtin_amdgcn_s_setprio(2);
+#pragma unroll
         for (int f = 0; f < 4; ++f) {
             if (f == 2 && !extra) continue;
             double WL[6], WR[6], pL, pR, du, dv, dw;
@@ -557,7 +558,7 @@ __device__ __forceinline__ double hydro_block(HydroLds& L, const Geom& g,
 }
 
 template <int STAGE, int RS>
-__global__ void __launch_bounds__(NT) hydro_stage_kernel(HydroStageArgs a)
+__global__ void __launch_bounds__(NT, 2) hydro_stage_kernel(HydroStageArgs a)
 {
     __shared__ HydroLds L;
     const Geom& g = a.g;
@@ -601,7 +602,7 @@ __device__ __forceinline__ void ens_entry(HydroLds& L, const EnsInst* __restrict
 // Batched stage over a table of (instance, block) entries, one workgroup per
 // entry; XCD_REMAP gives every XCD a contiguous run of the table.
 template <int STAGE, int XCD_REMAP>
-__global__ void __launch_bounds__(NT) ens_stage_kernel(const EnsInst* __restrict__ inst,
+__global__ void __launch_bounds__(NT, 2) ens_stage_kernel(const EnsInst* __restrict__ inst,
                                                       const EnsBlock* __restrict__ blocks)
 {
     __shared__ HydroLds L;
@@ -615,7 +616,7 @@ __global__ void __launch_bounds__(NT) ens_stage_kernel(const EnsInst* __restrict
 // (dispatched to XCD w mod 8) drains chunk w mod 8 first, then the others.
 // Every workgroup leaves once all counters have passed their chunk ends.
 template <int STAGE, int XCD_QUEUES>
-__global__ void __launch_bounds__(NT) ens_stage_persistent_kernel(const EnsInst* __restrict__ inst,
+__global__ void __launch_bounds__(NT, 2) ens_stage_persistent_kernel(const EnsInst* __restrict__ inst,
                                                                  const EnsBlock* __restrict__ blocks,
                                                                  int nblocks, int* queues)
 {

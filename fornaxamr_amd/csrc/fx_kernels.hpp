@@ -119,6 +119,8 @@ hipError_t launch_blend_mhd(const Geom& g, double* W, const double* Ws, double* 
 hipError_t launch_div_scalar(const double* src, double* dst, double div, hipStream_t s);
 
 // per-face / per-edge batched entry points (global-memory SoA arrays)
+hipError_t launch_mc_slope_batch(int n, const double* qm, const double* q0, const double* qp,
+                                double* slope, hipStream_t s);
 hipError_t launch_hllc_batch(int n, const double* WL, const double* WR, const double* pL,
                              const double* pR, const double* gL, const double* gR,
                              const double* du, const double* dv, const double* dw,

@@ -49,39 +49,39 @@ __device__ __forceinline__ int gmap(int dir, int lv)
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
-// MC limited slope (src/prj_reconstruct.h:355-375).
-__device__ __forceinline__ double mc_slope(double qm, double q0, double qp)
-{
-    double sl = q0 - qm;
-    double sr = qp - q0;
-    double c0 = 2.0 * sl;
-    double c1 = sl + 0.5 * (sr - sl);
-    double c2 = 2.0 * sr;
-    double mn = c0 < c1 ? c0 : c1;
-    double mx = c0 > c1 ? c0 : c1;
-    mn = mn < c2 ? mn : c2;
-    mx = mx > c2 ? mx : c2;
-    double slope = sl > 0.0 ? mn : mx;
-    return (sl * sr > 0.0) ? slope : 0.0;
-}
-
 // Half slope as the flux assembly uses it (src/prj_flux.c:410-418):
-// h = 0.5 * mc_slope(qm, q0, qp), evaluated as
+// h = 0.5 * prj_reconstruct_mc_face_slope(qm, q0, qp)
+// (src/prj_reconstruct.h:355-375), evaluated as
 //   (sl*sr > 0) ? 0.5 * copysign(min(|2 sl|, |c1|, |2 sr|), sl) : 0.
 // When sl*sr > 0 the three candidates c0 = 2 sl, c1 = sl + (sr - sl)/2 and
 // c2 = 2 sr are nonzero with the sign of sl, so min(c0,c1,c2) for sl > 0 and
 // max(c0,c1,c2) for sl < 0 both equal copysign(min|c|, sl) exactly (negation
 // and scaling by 2 are exact); otherwise both forms return +0.  Inputs with
-// NaN give sl*sr > 0 false and 0 in both.  The same bits, half the
-// instructions of the compare/select chain.
+// NaN give sl*sr > 0 false and 0 in both.  min(|2 sl|, |2 sr|) is taken as
+// 2 min(|sl|, |sr|): doubling is exact (or overflows alike) and monotone, and
+// a NaN c1 (from infinite slopes) is ignored by fmin in either order.  The
+// same bits, half the instructions of the compare/select chain.
 __device__ __forceinline__ double mc_half(double qm, double q0, double qp)
 {
     double sl = q0 - qm;
     double sr = qp - q0;
     double c1 = sl + 0.5 * (sr - sl);
-    double m = fmin(fmin(fabs(2.0 * sl), fabs(c1)), fabs(2.0 * sr));
+    double m = fmin(2.0 * fmin(fabs(sl), fabs(sr)), fabs(c1));
     double h = 0.5 * copysign(m, sl);
     return (sl * sr > 0.0) ? h : 0.0;
+}
+
+// The full MC face slope, prj_reconstruct_mc_face_slope (src/prj_reconstruct.h:
+// 355-375), in the form of mc_half above: mc_half(...) == 0.5 * mc_slope(...)
+// bit for bit (the same copysign(min|c|, sl), then the same product by 0.5).
+__device__ __forceinline__ double mc_slope(double qm, double q0, double qp)
+{
+    double sl = q0 - qm;
+    double sr = qp - q0;
+    double c1 = sl + 0.5 * (sr - sl);
+    double m = fmin(2.0 * fmin(fabs(sl), fabs(sr)), fabs(c1));
+    double s = copysign(m, sl);
+    return (sl * sr > 0.0) ? s : 0.0;
 }
 
 // Shock-detector pressure-jump limiter (src/prj_riemann.c:8-17).  The
@@ -101,6 +101,17 @@ __device__ __forceinline__ double dmin_delta(double a, double b) { return fmin(a
 
 // HLLC with the theta-limited contact speed (src/prj_riemann.c:1433-1599),
 // hydro build (6 variables).  W* are face states in the rotated frame.
+//
+// Written without branches: the reference returns F_L where 0 <= S_L, F_R
+// where S_R <= 0, and otherwise the star flux of the side the contact speed
+// selects (Toro 2009, eqs. 10.33-10.36).  Here the upwind side K is chosen
+// once -- L if 0 <= S_L, R if S_R <= 0, else by the sign of S_M -- and F_K
+// (formed from K's state by the same expressions as the reference's
+// supersonic fluxes) is taken as it is in the supersonic cases and as the base
+// of the star flux otherwise.  The star terms are evaluated in every case and
+// discarded by a select where the reference would not have formed them, so
+// every returned bit is the reference's; straight-line code lets the compiler
+// interleave the independent face solves of a cell.
 __device__ __forceinline__ void hllc(const double* WL, const double* WR, double pL, double pR,
                                      double gL, double gR, double du, double dv, double dw,
                                      double* flux, double* vface)
@@ -121,30 +132,9 @@ __device__ __forceinline__ void hllc(const double* WL, const double* WR, double 
     double SM = (theta * (pR - pL) + UL1 * (SL - uL) - UR1 * (SR - uR)) /
                 (rL * (SL - uL) - rR * (SR - uR));
 
-    if (0.0 <= SL) {
-        flux[0] = rL * uL;
-        flux[1] = rL * uL * uL + pL;
-        flux[2] = rL * uL * vL;
-        flux[3] = rL * uL * wL;
-        flux[4] = (eL + pL) * uL;
-        flux[5] = rL * yL * uL;
-        vface[0] = uL; vface[1] = vL; vface[2] = wL;
-        return;
-    }
-    if (SR <= 0.0) {
-        flux[0] = rR * uR;
-        flux[1] = rR * uR * uR + pR;
-        flux[2] = rR * uR * vR;
-        flux[3] = rR * uR * wR;
-        flux[4] = (eR + pR) * uR;
-        flux[5] = rR * yR * uR;
-        vface[0] = uR; vface[1] = vR; vface[2] = wR;
-        return;
-    }
-    vface[0] = SM;
-    // Select the upwind side once; the star-state formulas are identical in
-    // form for both sides (Toro 2009, eqs. 10.33-10.36).
-    const bool left = 0.0 <= SM;
+    const bool sup_l = 0.0 <= SL;
+    const bool sup_r = !sup_l && SR <= 0.0;
+    const bool left = sup_l || (!sup_r && 0.0 <= SM);
     double r = left ? rL : rR;
     double u = left ? uL : uR;
     double v = left ? vL : vR;
@@ -153,22 +143,33 @@ __device__ __forceinline__ void hllc(const double* WL, const double* WR, double 
     double e = left ? eL : eR;
     double p = left ? pL : pR;
     double S = left ? SL : SR;
-    vface[1] = v;
-    vface[2] = w;
-    double rho_star = r * (S - u) / (S - SM);
-    double e_over_rho = e / r + (SM - u) * (SM + p / (r * (S - u)));
     double Fr = r * u;
     double Fm1 = r * u * u + p;
     double Fm2 = r * u * v;
     double Fm3 = r * u * w;
     double Fe = (e + p) * u;
     double Fy = r * y * u;
-    flux[0] = Fr + S * (rho_star - r);
-    flux[1] = Fm1 + S * (rho_star * SM - r * u);
-    flux[2] = Fm2 + S * (rho_star * v - r * v);
-    flux[3] = Fm3 + S * (rho_star * w - r * w);
-    flux[4] = Fe + S * (rho_star * e_over_rho - e);
-    flux[5] = Fy + S * (rho_star * y - r * y);
+    double rho_star = r * (S - u) / (S - SM);
+    double e_over_rho = e / r + (SM - u) * (SM + p / (r * (S - u)));
+    // star fluxes formed before the selects, so that each select picks
+    // between two computed values (a conditional expression would be emitted
+    // as a branch around the star terms)
+    const double s0 = Fr + S * (rho_star - r);
+    const double s1 = Fm1 + S * (rho_star * SM - r * u);
+    const double s2 = Fm2 + S * (rho_star * v - r * v);
+    const double s3 = Fm3 + S * (rho_star * w - r * w);
+    const double s4 = Fe + S * (rho_star * e_over_rho - e);
+    const double s5 = Fy + S * (rho_star * y - r * y);
+    const bool sup = sup_l || sup_r;
+    flux[0] = sup ? Fr : s0;
+    flux[1] = sup ? Fm1 : s1;
+    flux[2] = sup ? Fm2 : s2;
+    flux[3] = sup ? Fm3 : s3;
+    flux[4] = sup ? Fe : s4;
+    flux[5] = sup ? Fy : s5;
+    vface[0] = sup ? u : SM;
+    vface[1] = v;
+    vface[2] = w;
 }
 
 // HLLE: the HLL flux (Harten, Lax & van Leer 1983) over the Davis wave-speed
@@ -196,28 +197,24 @@ __device__ __forceinline__ void hlle_hydro(const double* WL, const double* WR, d
                     rR * yR * uR};
     double SL = dmin(uL - csL, uR - csR);
     double SR = dmax(uL + csL, uR + csR);
-    if (0.0 <= SL) {
-#pragma unroll
-        for (int v = 0; v < 6; ++v) flux[v] = FL[v];
-        vface[0] = uL; vface[1] = vL; vface[2] = wL;
-        return;
-    }
-    if (SR <= 0.0) {
-#pragma unroll
-        for (int v = 0; v < 6; ++v) flux[v] = FR[v];
-        vface[0] = uR; vface[1] = vR; vface[2] = wR;
-        return;
-    }
+    // supersonic faces take the upwind flux; the fan terms are formed in
+    // every case and discarded there (straight-line code, same bits)
+    const bool sup_l = 0.0 <= SL;
+    const bool sup_r = !sup_l && SR <= 0.0;
     const double inv = 1.0 / (SR - SL);
     double Uh[4];
 #pragma unroll
     for (int v = 0; v < 6; ++v) {
-        flux[v] = (SR * FL[v] - SL * FR[v] + SL * SR * (UR[v] - UL[v])) * inv;
+        const double hll = (SR * FL[v] - SL * FR[v] + SL * SR * (UR[v] - UL[v])) * inv;
+        const double up = sup_l ? FL[v] : FR[v];
+        flux[v] = (sup_l || sup_r) ? up : hll;
         if (v < 4) Uh[v] = (SR * UR[v] - SL * UL[v] + FL[v] - FR[v]) * inv;
     }
-    vface[0] = Uh[1] / Uh[0];
-    vface[1] = Uh[2] / Uh[0];
-    vface[2] = Uh[3] / Uh[0];
+    const double vh1 = Uh[1] / Uh[0], vh2 = Uh[2] / Uh[0], vh3 = Uh[3] / Uh[0];
+    const bool sup = sup_l || sup_r;
+    vface[0] = sup ? (sup_l ? uL : uR) : vh1;
+    vface[1] = sup ? (sup_l ? vL : vR) : vh2;
+    vface[2] = sup ? (sup_l ? wL : wR) : vh3;
 }
 
 // ---- HLLD (src/prj_riemann.c:19-440) --------------------------------------

@@ -245,6 +245,15 @@ __global__ void ens_dt_finalize_kernel(const EnsInst* __restrict__ inst, int nin
     }
 }
 
+// prj_reconstruct_mc_face_slope over n stencils (qm, q0, qp)
+__global__ void mc_slope_batch_kernel(int n, const double* qm, const double* q0, const double* qp,
+                                      double* slope)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    slope[f] = mc_slope(qm[f], q0[f], qp[f]);
+}
+
 __global__ void hllc_batch_kernel(int n, const double* WL, const double* WR, const double* pL,
                                   const double* pR, const double* gL, const double* gR,
                                   const double* du, const double* dv, const double* dw,
@@ -652,6 +661,15 @@ hipError_t launch_ens_dt_finalize(const EnsInst* inst, int ninst, double cfl, do
 {
     hipLaunchKernelGGL(ens_dt_finalize_kernel, dim3(grid_for(ninst, 64)), dim3(64), 0, s, inst,
                        ninst, cfl, factor);
+    return hipGetLastError();
+}
+
+hipError_t launch_mc_slope_batch(int n, const double* qm, const double* q0, const double* qp,
+                                double* slope, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mc_slope_batch_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, qm, q0,
+                       qp, slope);
     return hipGetLastError();
 }
 

@@ -100,6 +100,12 @@ class GpuMesh:
         (Ketcheson's eSSPRK(10,4), the reference's default; hydro and MHD)."""
         _native.check(self.lib.fx_mesh_set_integrator(self.h, self.INTEGRATORS[name]))
 
+    def set_mhd_schedule(self, ct_split=0, side_stream=-1):
+        """MHD launch schedule: CT workgroups per block (1, 2, 4; 0 automatic)
+        and the ghost-layer face kernel beside the flux kernel (1), in order
+        (0) or automatic (-1).  Every schedule gives the same bits."""
+        _native.check(self.lib.fx_mesh_set_mhd_schedule(self.h, int(ct_split), int(side_stream)))
+
     RIEMANN = {"hllc": 0, "hlle": 1}
 
     def set_riemann(self, name):

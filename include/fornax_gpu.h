@@ -96,6 +96,14 @@ int fx_mesh_set_integrator(fx_mesh *m, int integrator);
  * face states, in the form of the reference's HLL fallback
  * (src/prj_riemann.c:393-407); the reference has no hydro HLLE, so this
  * option has no reference counterpart.  MHD meshes take HLLD only. */
+/* Launch schedule of an MHD mesh's stage (no reference counterpart; every
+ * schedule gives the same bits).  ct_split: CT workgroups per block, 1, 2 or
+ * 4 (each marches 16/ct_split of the block's x-planes), 0 automatic (more
+ * where the blocks would not fill the chip's CT slots).  side_stream: 1 runs
+ * the ghost-layer face kernel on a second stream beside the flux kernel, 0 in
+ * order on the mesh's stream, -1 automatic (beside it where the flux kernel
+ * fits the chip in one round).  Synchronises the mesh's streams first. */
+int fx_mesh_set_mhd_schedule(fx_mesh *m, int ct_split, int side_stream);
 #define FX_RIEMANN_HLLC 0
 #define FX_RIEMANN_HLLE 1
 int fx_mesh_set_riemann(fx_mesh *m, int solver);
@@ -129,6 +137,11 @@ int fx_mesh_view_get(fx_mesh *m, fx_mesh_view *out);
 
 /* ---- per-face / per-edge entry points on device arrays (SoA: component c
  *      of item i at p[c*n + i]); stream may be NULL -------------------------- */
+/* prj_reconstruct_mc_face_slope (src/prj_reconstruct.h:355-375), the MC
+ * limited slope of the 3-cell stencil (qm, q0, qp), over n stencils of device
+ * arrays; the face values are q0 +/- 0.5 * slope. */
+int fx_reconstruct_mc_slope_batch(int n, const double *qm, const double *q0, const double *qp,
+    double *slope, void *stream);
 /* prj_riemann_hllc (src/prj_riemann.h:7-10): WL/WR 6 rotated prims; flux 6,
  * v_face 3. */
 int fx_riemann_hllc_batch(int n, const double *WL, const double *WR, const double *pL,

@@ -290,6 +290,7 @@ def ref_lib(mhd, key=None):
         lib.refh_stage.argtypes = [C.c_int, C.c_double]
         lib.refh_step.restype = C.c_double
         lib.refh_step.argtypes = [C.c_double, C.c_double]
+        lib.refh_mc_face_slope.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         lib.prj_riemann_hllc.argtypes = [_dp, _dp] + [C.c_double] * 4 + [C.c_void_p, _dp, _dp] + \
             [C.c_double] * 3
         if mhd:

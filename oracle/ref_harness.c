@@ -408,6 +408,20 @@ double refh_emf_upwind(int dir, const double *emf_face, const double *emf_cell,
     return prj_mhd_emf_upwind(b, dir, 0, 0, 0, emf_face, emf_cell, v_norm, 0, 0.0);
 }
 
+#endif
+
+/* The reference's MC face slope (a static inline function of
+ * src/prj_reconstruct.h) over n stencils. */
+void refh_mc_face_slope(int n, const double *qm, const double *q0, const double *qp,
+    double *slope)
+{
+    int i;
+    for (i = 0; i < n; ++i) {
+        slope[i] = prj_reconstruct_mc_face_slope(qm[i], q0[i], qp[i]);
+    }
+}
+
+#if PRJ_MHD
 /* The same with the resistive term: edge (dir, i, j, k) of the scratch block
  * with its cell widths set to dx, face fields bf[3] (FACE_IDX arrays of
  * NFACES), resistivity eta.  Returns the value; *stored receives what the

@@ -2,8 +2,9 @@
 
 Runs in the development container only (it needs oracle/_ref, which
 oracle/Makefile compiles from the reference sources).  Fixtures are data:
-per-face / per-edge input/output vectors of the reference's own
-prj_riemann_hllc, prj_riemann_hlld and prj_mhd_emf_upwind, and, for whole
+per-stencil / per-face / per-edge input/output vectors of the reference's own
+prj_reconstruct_mc_face_slope, prj_riemann_hllc, prj_riemann_hlld and
+prj_mhd_emf_upwind, and, for whole
 RK2 trajectories, the SHA-256 of the input state and of the state after every
 step plus the dt sequence and a few sampled values; for the per-block entry
 points, the SHA-256 of the reference's prj_flux_update / prj_flux_div outputs
@@ -14,7 +15,7 @@ reference's prj_riemann_flux_send / prj_mhd_emf_send outputs on seeded
 block arrays (cf_corrections.json).
 
     python tests/golden/make_golden.py [--only-block | --only-cf | --only-emf-block |
-                                        --only-trajectories | --only-bf2bc]
+                                        --only-trajectories | --only-bf2bc | --only-mc]
 """
 from __future__ import annotations
 
@@ -173,6 +174,33 @@ TRAJECTORIES = {
 }
 
 
+def mc_vectors(lib, rng, n=4096):
+    """prj_reconstruct_mc_face_slope on stencils (qm, q0, qp): random smooth
+    and rough data, and the special cases of the limiter -- equal neighbours,
+    sign changes, ties between the three candidates, signed zeros,
+    subnormal differences, differences that overflow when doubled,
+    infinities and NaN."""
+    q = rng.normal(0.0, 1.0, (n, 3))
+    q[: n // 8] = np.cumsum(rng.uniform(0.0, 1.0, (n // 8, 3)), axis=1)     # monotone up
+    q[n // 8: n // 4] = -np.cumsum(rng.uniform(0.0, 1.0, (n // 8, 3)), axis=1)
+    special = []
+    tiny, huge, inf, nan = 5e-324, 1.7e308, np.inf, np.nan
+    vals = [0.0, -0.0, 1.0, -1.0, 2.0, 3.0, tiny, -tiny, 1e-310, -1e-310, huge, -huge, inf, -inf,
+            nan, 0.5, 1.5, 1e-300]
+    for a in vals:
+        for b in vals:
+            for c in vals:
+                special.append((a, b, c))
+    special += [(0.0, 1.0, 3.0), (0.0, 1.0, 1.5), (0.0, 2.0, 3.0), (3.0, 1.0, 0.0),
+                (1.0, 1.0, 2.0), (0.0, 0.0, 0.0), (-0.0, 0.0, -0.0)]
+    q = np.concatenate([q, np.array(special)])
+    qm, q0, qp = (np.ascontiguousarray(q[:, i]) for i in range(3))
+    out = np.empty(len(q))
+    lib.refh_mc_face_slope(len(q), qm.ctypes.data_as(_dp), q0.ctypes.data_as(_dp),
+                           qp.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return dict(qm=qm, q0=q0, qp=qp, slope=out)
+
+
 def make_problem(name):
     fac, kw, steps = TRAJECTORIES[name]
     return getattr(P, fac)(**kw), steps
@@ -289,6 +317,10 @@ def main():
         np.savez_compressed(os.path.join(HERE, "emf_block_vectors.npz"),
                             **emf_block_vectors(O.ref_lib(True), np.random.default_rng(20261203)))
         return
+    if "--only-mc" in sys.argv:
+        np.savez_compressed(os.path.join(HERE, "mc_vectors.npz"),
+                            **mc_vectors(O.ref_lib(False), np.random.default_rng(20261204)))
+        return
     if "--only-trajectories" in sys.argv:
         recs = {name: trajectory(name) for name in TRAJECTORIES}
         with open(os.path.join(HERE, "trajectories.json"), "w") as f:
@@ -312,6 +344,8 @@ def main():
     np.savez(os.path.join(HERE, "emf_vectors.npz"), **emf_vectors(libm, rng))
     np.savez_compressed(os.path.join(HERE, "emf_block_vectors.npz"),
                         **emf_block_vectors(libm, np.random.default_rng(20261203)))
+    np.savez_compressed(os.path.join(HERE, "mc_vectors.npz"),
+                        **mc_vectors(libh, np.random.default_rng(20261204)))
     recs = {name: trajectory(name) for name in TRAJECTORIES}
     with open(os.path.join(HERE, "trajectories.json"), "w") as f:
         json.dump(recs, f, indent=1)

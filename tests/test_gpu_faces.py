@@ -72,6 +72,22 @@ def test_hlld_batch_reports_bad_state(gpu):
     assert st.cpu().tolist() == [0, 11, 12, 0]
 
 
+def test_mc_slope_batch(gpu):
+    """prj_reconstruct_mc_face_slope on the GPU (the form the stage kernels'
+    half slopes take) against the reference's vectors, special cases
+    included: identical bits, signs of zeros included."""
+    import torch
+    d = np.load(os.path.join(GOLD, "mc_vectors.npz"))
+    n = d["slope"].shape[0]
+    ins = [_dev(d[k]) for k in ("qm", "q0", "qp")]
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    rc = gpu.fx_reconstruct_mc_slope_batch(n, *[a.data_ptr() for a in ins], out.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), np.ascontiguousarray(d["slope"]).view(np.uint64))
+
+
 def test_emf_batch(gpu):
     import torch
     d = np.load(os.path.join(GOLD, "emf_vectors.npz"))

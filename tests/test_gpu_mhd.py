@@ -56,6 +56,59 @@ def test_resistive_ct_steps(gpu, case):
                 assert np.array_equal(a, b), (s, st)
 
 
+@pytest.mark.parametrize("ct_split", [1, 2, 4])
+@pytest.mark.parametrize("side", [0, 1])
+def test_schedules_match_oracle(gpu, ct_split, side):
+    """Every CT split (one block's planes over 1, 2 or 4 workgroups, each with
+    its own prologue) and the ghost-layer face kernel in order or on the side
+    stream: cells, faces and dt bit-identical to the oracle over two steps,
+    stage by stage, on a resistive mesh (the edges' curl B reads faces the CT
+    updates)."""
+    pr = P.random_smooth(n=(32, 48, 32), seed=44, bc=(0,) * 6, mhd=True, eta=1.0e-3)
+    gm, om = _pair(pr)
+    gm.set_mhd_schedule(ct_split, side)
+    for s in range(2):
+        dt = om.calc_dt(pr.cfl)
+        assert gm.calc_dt(pr.cfl) == dt, s
+        for st in (1, 2):
+            gm.stage(st, dt)
+            om.stage(st, dt)
+            gm.sync()
+            slot = 1 if st == 1 else 0
+            assert ulp_diff(gm.get_prim(slot), om.W1 if st == 1 else om.W0) == 0, (s, st)
+            for a, b in zip(gm.get_bf(slot), om.bf1 if st == 1 else om.bf0):
+                assert np.array_equal(a, b), (s, st)
+    # the step path with recorded CFL maxima per CT workgroup of this split
+    for s in range(2):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    assert ulp_diff(gm.get_prim(0), om.W0) == 0
+    gm.close()
+
+
+def test_large_mesh_default_schedule(gpu):
+    """144^3 (729 blocks: more than fit the chip in one round, so the default
+    schedule is CT split 1 with the ghost-layer faces in order, as at 256^3):
+    two steps bit-identical to the oracle."""
+    pr = P.magnetized_sedov(n=(144, 144, 144))
+    gm, om = _pair(pr)
+    for s in range(2):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    assert ulp_diff(gm.get_prim(0), om.W0) == 0
+    for a, b in zip(gm.get_bf(0), om.bf0):
+        assert np.array_equal(a, b)
+    gm.close()
+
+
+def test_schedule_arguments(gpu):
+    from fornaxamr_amd import _native
+    pr = P.random_smooth(n=(32, 32, 32), seed=45, bc=(0,) * 6, mhd=True)
+    gm, _ = _pair(pr)
+    for bad in ((3, 0), (8, 0), (1, 2)):
+        with pytest.raises(_native.FxError):
+            gm.set_mhd_schedule(*bad)
+    gm.close()
+
+
 def test_single_stage(gpu):
     pr = P.random_smooth(n=(32, 32, 32), seed=41, bc=(0,) * 6, mhd=True)
     gm, om = _pair(pr)

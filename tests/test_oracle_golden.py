@@ -1,8 +1,8 @@
 """The C restatement against the committed golden fixtures (reference outputs).
 
 Fixtures come from tests/golden/make_golden.py (serial reference build): the
-reference's own per-face HLLC/HLLD and per-edge EMF functions on random
-states, and SHA-256 digests of whole RK2 trajectories.
+reference's own MC face slope, per-face HLLC/HLLD and per-edge EMF functions
+on random states, and SHA-256 digests of whole RK2 trajectories.
 """
 import ctypes as C
 import json
@@ -21,6 +21,25 @@ _dp = C.POINTER(C.c_double)
 
 def _p(a):
     return np.ascontiguousarray(a).ctypes.data_as(_dp)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def test_mc_slope_vectors():
+    """The oracle's MC face slope against the reference's on random stencils
+    and the limiter's special cases (signed zeros, subnormals, overflow when
+    doubled, infinities, NaN): the same bits, signs of zeros included."""
+    d = np.load(os.path.join(GOLD, "mc_vectors.npz"))
+    lib = O.oracle_lib()
+    got = np.array([lib.fxo_mc_slope(a, b, c) for a, b, c in zip(d["qm"], d["q0"], d["qp"])])
+    assert np.array_equal(_bits(got), _bits(d["slope"]))
+    if O.ref_available("hydro"):
+        ref = np.empty_like(got)
+        O.ref_lib(False).refh_mc_face_slope(len(got), _p(d["qm"]), _p(d["q0"]), _p(d["qp"]),
+                                            _p(ref))
+        assert np.array_equal(_bits(ref), _bits(d["slope"]))
 
 
 def test_hllc_vectors():
