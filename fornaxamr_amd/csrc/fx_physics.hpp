@@ -479,15 +479,18 @@ __device__ __forceinline__ double emf_upwind(const double* ef, const double* ec,
 // correctly rounded log_cr (fx_log.hpp); at or below 1 every faithful log is
 // <= 0 and the factor is 0, so the log is skipped there.  The reference aborts
 // on a non-finite mask; none arises: each factor is at most
-// log(DBL_MAX) < 710, so the mask stays below 1e12 * 710^2.
+// log(DBL_MAX) < 710, so the mask stays below 1e12 * 710^2.  The quotients
+// are formed only past the thresholds: rounding is monotone, so rho <= 1e13
+// gives rho / 1e13 <= 1 (and ye / 0.25 is exact), and the factors are 0
+// there without a division.
 __device__ __forceinline__ double eta_mask(double rho, double ye)
 {
     double rf = 0.0, yf = 0.0;
-    if (isfinite(rho) && rho > 0.0) {
+    if (rho > 1.0e13 && isfinite(rho)) {
         double a = rho / 1.0e13;
         if (a > 1.0) rf = log_cr(a);
     }
-    if (isfinite(ye) && ye > 0.0) {
+    if (ye > 0.25 && isfinite(ye)) {
         double b = ye / 0.25;
         if (b > 1.0) yf = log_cr(b);
     }
