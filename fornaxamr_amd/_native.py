@@ -84,6 +84,7 @@ def load():
     lib.fx_stage.argtypes = [vp, C.c_int, C.c_double]
     lib.fx_step_async.argtypes = [vp, C.c_double, C.c_double]
     lib.fx_step.argtypes = [vp, C.c_double, C.c_double, _dp]
+    lib.fx_step_dt.argtypes = [vp, C.c_double]
     lib.fx_sync.argtypes = [vp]
     lib.fx_last_dt.argtypes = [vp, _dp]
     lib.fx_mesh_view_get.argtypes = [vp, C.POINTER(MeshView)]
@@ -139,7 +140,7 @@ def check(rc):
 EXPORTS = [
     "fx_mesh_create", "fx_mesh_destroy", "fx_mesh_set_stream", "fx_mesh_set_prim",
     "fx_mesh_get_prim", "fx_mesh_set_bf", "fx_mesh_get_bf", "fx_calc_dt", "fx_stage",
-    "fx_step_async", "fx_step", "fx_sync", "fx_last_dt", "fx_mesh_view_get",
+    "fx_step_async", "fx_step", "fx_step_dt", "fx_sync", "fx_last_dt", "fx_mesh_view_get",
     "fx_mesh_set_timing", "fx_mesh_kernel_time",
     "fx_reconstruct_mc_slope_batch", "fx_riemann_hllc_batch", "fx_riemann_hlld_batch",
     "fx_emf_upwind_batch",

@@ -8,6 +8,7 @@
 // except the calls documented to do so.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -423,11 +424,14 @@ int fx_mesh_get_bf(fx_mesh* m, int slot, double* bx, double* by, double* bz)
 int fx_calc_dt(fx_mesh* m, double cfl, double* dt)
 {
     if (int rc = check_mesh(m)) return rc;
+    // the integrator's scale of the CFL minimum (src/prj_timeint.c:2248-2253)
+    const double scale = m->integrator == FX_ESSPRK10_4 ? 6.0 : 1.0;
     if (m->cfl_ready)
-        FX_HIP(launch_dt_final(m->dev + 8, m->cfl_nparts, cfl, m->dev + 3, nullptr, 1.0, 1.0,
+        FX_HIP(launch_dt_final(m->dev + 8, m->cfl_nparts, cfl, m->dev + 3, nullptr, 1.0, scale,
                                m->stream));
     else
-        FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 3, nullptr, 1.0, m->stream));
+        FX_HIP(launch_calc_dt(m->g, m->W[0], cfl, m->dev + 8, m->dev + 3, nullptr, 1.0, m->stream,
+                              scale));
     double h = 0.0;
     FX_HIP(hipMemcpyAsync(&h, m->dev + 3, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     FX_HIP(hipStreamSynchronize(m->stream));
@@ -573,6 +577,24 @@ int fx_step_async(fx_mesh* m, double cfl, double dt_factor)
     }
     if (int rc = run_stage(m, 1, m->dev, 0.0)) return rc;
     return run_stage(m, 2, m->dev, 0.0);
+}
+
+int fx_step_dt(fx_mesh* m, double dt)
+{
+    if (int rc = check_mesh(m)) return rc;
+    if (!(dt > 0.0) || !std::isfinite(dt))
+        return fail(FX_EINVAL, "fx_step_dt: dt must be finite and > 0");
+    // dt and the growth limit's previous dt of a later fx_step
+    const double h[2] = {dt, dt};
+    FX_HIP(hipMemcpyAsync(m->dev, h, sizeof(h), hipMemcpyHostToDevice, m->stream));
+    if (m->integrator == FX_ESSPRK10_4) {
+        m->cfl_ready = false;
+        if (int rc = essprk10_4_step(m)) return rc;
+    } else {
+        if (int rc = run_stage(m, 1, m->dev, 0.0)) return rc;
+        if (int rc = run_stage(m, 2, m->dev, 0.0)) return rc;
+    }
+    return fx_sync(m);
 }
 
 int fx_step(fx_mesh* m, double cfl, double dt_factor, double* dt_used)

@@ -118,6 +118,10 @@ class GpuMesh:
         _native.check(self.lib.fx_step(self.h, float(cfl), float(dt_factor), C.byref(dt)))
         return dt.value
 
+    def step_dt(self, dt):
+        """One step of the integrator with the caller's dt (synchronous)."""
+        _native.check(self.lib.fx_step_dt(self.h, float(dt)))
+
     def step_async(self, cfl, dt_factor=1.2):
         _native.check(self.lib.fx_step_async(self.h, float(cfl), float(dt_factor)))
 

@@ -62,7 +62,8 @@ int fx_mesh_set_bf(fx_mesh *m, const double *bx, const double *by, const double 
 int fx_mesh_get_bf(fx_mesh *m, int slot, double *bx, double *by, double *bz);
 
 /* ---- the RK2 path -------------------------------------------------------- */
-/* CFL step of slot 0 (prj_timeint_calc_dt, src/prj_timeint.c:2144-2256). */
+/* CFL step of slot 0 (prj_timeint_calc_dt, src/prj_timeint.c:2144-2256),
+ * with the integrator's factor: x 6 on an eSSPRK(10,4) mesh. */
 int fx_calc_dt(fx_mesh *m, double cfl, double *dt);
 /* One RK2 stage over all blocks: the body of prj_timeint_stage1/2
  * (src/prj_timeint.c:2732-2973) -- domain ghost fill, reconstruction,
@@ -79,6 +80,13 @@ int fx_stage(fx_mesh *m, int stage, double dt);
 int fx_step_async(fx_mesh *m, double cfl, double dt_factor);
 /* Same, then synchronises and returns the dt used. */
 int fx_step(fx_mesh *m, double cfl, double dt_factor, double *dt_used);
+/* One step of the mesh's integrator with the caller's dt: prj_timeint_step
+ * (src/prj_timeint.c:3221-3283) called with dt_step, for a main loop that
+ * forms dt on the host as src/main.c:510-545 does -- fx_calc_dt, the growth
+ * limit, then dt clipped to the next output/event time and to t_end.  A later
+ * fx_step takes this dt as its previous one.  Synchronises; dt must be finite
+ * and > 0 (FX_EINVAL otherwise). */
+int fx_step_dt(fx_mesh *m, double dt);
 /* Time integrator of fx_step / fx_step_async (TIME_INTEGRATION of
  * src/prj_defs.h:136-176).  FX_RK2 (default): the two stages above.
  * FX_ESSPRK10_4: Ketcheson's eSSPRK(10,4), the reference's default, as in

@@ -290,6 +290,8 @@ def ref_lib(mhd, key=None):
         lib.refh_stage.argtypes = [C.c_int, C.c_double]
         lib.refh_step.restype = C.c_double
         lib.refh_step.argtypes = [C.c_double, C.c_double]
+        lib.refh_step_until.restype = C.c_double
+        lib.refh_step_until.argtypes = [C.c_double] * 3
         lib.refh_mc_face_slope.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         lib.prj_riemann_hllc.argtypes = [_dp, _dp] + [C.c_double] * 4 + [C.c_void_p, _dp, _dp] + \
             [C.c_double] * 3
@@ -303,6 +305,9 @@ def ref_lib(mhd, key=None):
             lib.refh_emf_upwind_block.argtypes = [C.c_int] * 4 + [_dp] * 7 + [C.c_double, _dp]
             lib.prj_riemann_hlld.argtypes = [_dp, _dp] + [C.c_double] * 4 + \
                 [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp] + [C.c_double] * 3
+        if key.endswith("_fxflux"):
+            lib.refh_fx_step_until.restype = C.c_double
+            lib.refh_fx_step_until.argtypes = [C.c_double] * 3
         _ref_libs[key] = lib
     return _ref_libs[key]
 
@@ -310,8 +315,10 @@ def ref_lib(mhd, key=None):
 class RefSim:
     """The serial reference build driven like src/main.c (one mesh per process).
     key selects the build: "hydro"/"mhd" (RK2), "hydro_essprk10_4" /
-    "mhd_essprk10_4" (eSSPRK(10,4)), or "hydro_fxflux"/"mhd_fxflux" (RK2 with
-    prj_flux_update on the GPU through the reference-side shim)."""
+    "mhd_essprk10_4" (eSSPRK(10,4)), or either of those with "_fxflux"
+    appended: the same build with the reference-side shims linked to the GPU
+    library (prj_flux_update on the GPU; the fx_* methods run the mesh-level
+    step on the GPU)."""
 
     def __init__(self, n, xlim, bc, gamma, mhd=False, eta=0.0, key=None):
         self.lib = ref_lib(mhd, key)
@@ -361,8 +368,22 @@ class RefSim:
     def stage(self, s, dt):
         self.lib.refh_stage(int(s), float(dt))
 
-    def step(self, cfl, dt_factor=1.2):
-        return self.lib.refh_step(float(cfl), float(dt_factor))
+    def step(self, cfl, dt_factor=1.2, t_end=1.0e300):
+        """One main-loop step; its dt is clipped so that the run ends at t_end."""
+        return self.lib.refh_step_until(float(cfl), float(dt_factor), float(t_end))
+
+    # mesh-level shim (fornaxamr_amd/shim/prj_timeint_fx.c), *_fxflux builds
+    def fx_attach(self):
+        self.lib.refh_fx_attach()
+
+    def fx_step(self, cfl, dt_factor=1.2, t_end=1.0e300):
+        return self.lib.refh_fx_step_until(float(cfl), float(dt_factor), float(t_end))
+
+    def fx_download(self):
+        self.lib.refh_fx_download()
+
+    def fx_detach(self):
+        self.lib.refh_fx_detach()
 
     def close(self):
         self.lib.refh_destroy()

@@ -381,22 +381,67 @@ void refh_stage(int stage, double dt)
 
 /* One main-loop step: dt from the CFL condition with the dt_factor growth
  * limit (src/main.c:510-522), then the RK2 step.  Returns the dt used. */
-double refh_step(double cfl, double dt_factor)
+/* main.c's growth limit of sim.dt, then dt_step clipped to t_end
+ * (src/main.c:517-535); returns dt_step */
+static double refh_limit_and_clip(double dt_new, double dt_factor, double t_end)
 {
-    double dt_new = refh_calc_dt(cfl);
-    double dt_src = 1.0e100;
+    double dt_step;
     if (g_sim.dt > 0.0) {
         double dt_limit = dt_factor * g_sim.dt;
         g_sim.dt = dt_new <= dt_limit ? dt_new : dt_limit;
     } else {
         g_sim.dt = dt_new;
     }
+    dt_step = g_sim.dt;
+    if (g_sim.time + dt_step > t_end) {
+        dt_step = t_end - g_sim.time;
+    }
+    return dt_step;
+}
+
+/* one main-loop step with the run ending at t_end (src/main.c:510-549) */
+double refh_step_until(double cfl, double dt_factor, double t_end)
+{
+    double dt_src = 1.0e100;
+    double dt_step = refh_limit_and_clip(refh_calc_dt(cfl), dt_factor, t_end);
     g_sim.mesh.time_seconds = g_sim.time;
     prj_timeint_step(&g_sim.mesh, &g_sim.coord, &g_sim.bc, &g_sim.eos, &g_sim.rad,
-        &g_sim.grav, &g_mpi, &PRJ_TIMEINT_TABLEAU_NAME, g_sim.dt, &dt_src, 0);
-    g_sim.time += g_sim.dt;
-    return g_sim.dt;
+        &g_sim.grav, &g_mpi, &PRJ_TIMEINT_TABLEAU_NAME, dt_step, &dt_src, 0);
+    g_sim.time += dt_step;
+    return dt_step;
 }
+
+double refh_step(double cfl, double dt_factor)
+{
+    return refh_step_until(cfl, dt_factor, 1.0e300);
+}
+
+#ifdef REFH_FX_MESH
+/* The mesh-level drop-in (fornaxamr_amd/shim/prj_timeint_fx.c) on this mesh:
+ * attach after refh_prepare, step on the GPU, download into the blocks. */
+void prj_fx_attach(prj_sim *sim);
+double prj_fx_calc_dt(prj_sim *sim);
+void prj_fx_step(prj_sim *sim, double dt_step);
+void prj_fx_download(prj_sim *sim, prj_mpi *mpi);
+void prj_fx_detach(void);
+
+void refh_fx_attach(void) { prj_fx_attach(&g_sim); }
+
+/* refh_step_until with the two reference calls replaced by the shim's */
+double refh_fx_step_until(double cfl, double dt_factor, double t_end)
+{
+    double dt_step;
+    g_sim.cfl = cfl;
+    dt_step = refh_limit_and_clip(prj_fx_calc_dt(&g_sim), dt_factor, t_end);
+    prj_fx_step(&g_sim, dt_step);
+    g_sim.time += dt_step;
+    return dt_step;
+}
+
+void refh_fx_download(void) { prj_fx_download(&g_sim, &g_mpi); }
+
+void refh_fx_detach(void) { prj_fx_detach(); }
+#endif
 
 /* Per-edge EMF entry (src/prj_mhd.c:627-718) on a scratch block, so tests can
  * call the reference upwind average with arbitrary inputs. */

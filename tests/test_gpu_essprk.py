@@ -51,6 +51,42 @@ def test_essprk_sedov_and_conservation(gpu):
     gm.close()
 
 
+@pytest.mark.parametrize("integrator", ["rk2", "essprk10_4"])
+@pytest.mark.parametrize("mhd", [False, True], ids=["hydro", "mhd"])
+def test_host_formed_dt_steps(gpu, integrator, mhd):
+    """fx_calc_dt + the main loop's growth limit on the host + fx_step_dt (the
+    shape of src/main.c:510-545) give the bits and dts of fx_step; fx_calc_dt
+    carries the integrator's factor (x 6 for eSSPRK(10,4),
+    src/prj_timeint.c:2248-2253)."""
+    from fornaxamr_amd.mesh import GpuMesh
+    pr = P.random_smooth(n=(32, 32, 48), seed=76, bc=(0,) * 6, mhd=mhd)
+    a = GpuMesh.from_problem(pr)
+    b = GpuMesh.from_problem(pr)
+    rk = GpuMesh.from_problem(pr)
+    a.set_integrator(integrator)
+    b.set_integrator(integrator)
+    factor = 6.0 if integrator == "essprk10_4" else 1.0
+    assert b.calc_dt(pr.cfl) == factor * rk.calc_dt(pr.cfl)
+    dt_prev, dt_factor = 0.0, 1.05
+    for s in range(3):
+        dt_a = a.step(pr.cfl, dt_factor)
+        dt_new = b.calc_dt(pr.cfl)
+        if dt_prev > 0.0:
+            lim = dt_factor * dt_prev
+            dt_new = dt_new if dt_new <= lim else lim
+        b.step_dt(dt_new)
+        dt_prev = dt_new
+        assert dt_a == dt_new, s
+        assert np.array_equal(a.get_prim(0), b.get_prim(0)), s
+        if mhd:
+            for fa, fb in zip(a.get_bf(0), b.get_bf(0)):
+                assert np.array_equal(fa, fb), s
+    with pytest.raises(Exception):
+        b.step_dt(0.0)
+    for m in (a, b, rk):
+        m.close()
+
+
 MHD_CASES = {
     "random_outflow": lambda: P.random_smooth(n=(32, 32, 32), seed=74, bc=(0,) * 6, mhd=True),
     "sedov": lambda: P.magnetized_sedov(n=(32, 48, 32)),
