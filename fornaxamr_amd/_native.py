@@ -72,6 +72,8 @@ def load():
     lib = C.CDLL(path)
     vp = C.c_void_p
     lib.fx_last_error.restype = C.c_char_p
+    lib.fx_status_message.restype = C.c_char_p
+    lib.fx_status_message.argtypes = [C.c_int]
     lib.fx_version.restype = C.c_char_p
     lib.fx_mesh_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
     lib.fx_mesh_destroy.argtypes = [vp]
@@ -155,5 +157,5 @@ EXPORTS = [
     "fx_mesh_set_integrator", "fx_mesh_set_riemann", "fx_mesh_set_mhd_schedule",
     "fx_cf_plan_create", "fx_cf_plan_info", "fx_cf_plan_destroy", "fx_cf_flux_send",
     "fx_cf_emf_send",
-    "fx_last_error", "fx_version",
+    "fx_last_error", "fx_status_message", "fx_version",
 ]

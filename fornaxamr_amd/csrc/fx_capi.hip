@@ -89,9 +89,15 @@ const char* phys_message(int code)
     case 18: return "prj_riemann_hlld: outer wave coincides with contact wave";
     case 19: return "prj_riemann_hlld: invalid star density";
     case 20: return "prj_riemann_hlld: invalid double-star density weights";
+    case 31: return "prj_mhd_emf_upwind: non-finite input";
     case 32: return "prj_timeint_mhd_cell_emf_all: non-finite cell emf";
     case 33: return "prj_mhd_emf_upwind: non-finite input or result";
     case 34: return "prj_timeint_mhd_ct_value: non-finite updated magnetic field";
+    case 35: return "prj_mhd_emf_upwind: resistive curl B reads outside the face storage";
+    case 36: return "prj_mhd_emf_upwind: non-finite resistive curl B";
+    case 37: return "prj_mhd_emf_upwind: edge index is out of storage bounds";
+    case 38: return "prj_mhd_emf_upwind: null face field for resistive term";
+    case 39: return "prj_mhd_emf_upwind: non-finite result";
     default: return "device error";
     }
 }
@@ -255,6 +261,8 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
 extern "C" {
 
 const char* fx_last_error(void) { return g_err.c_str(); }
+
+const char* fx_status_message(int status) { return status == 0 ? "ok" : phys_message(status); }
 
 const char* fx_version(void)
 {

@@ -349,7 +349,8 @@ __device__ __forceinline__ bool block_face_ok(int d, int i, int j, int k)
 // edge of one block per thread.  status: 0, 37 edge outside the block's edge
 // storage (src/prj_mhd.c:636-640), 31 non-finite input, 38 a face array
 // missing for a nonzero eta, 35 a curl read outside the block's face storage
-// (out of bounds in the reference), 36 non-finite curl B or result.
+// (out of bounds in the reference), 36 non-finite curl B, 39 non-finite result
+// (src/prj_mhd.c:707-715).
 __global__ void emf_block_batch_kernel(EmfBlockArgs a)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,7 +396,7 @@ __global__ void emf_block_batch_kernel(EmfBlockArgs a)
                     else e += eta * curl_b;
                 }
             }
-            if (st == 0 && !isfinite(e)) st = 36;
+            if (st == 0 && !isfinite(e)) st = 39;
             if (st == 0) {
                 emf = e;
                 if (a.block_emf[dir] != nullptr) a.block_emf[dir][block_edge_idx(dir, i, j, k)] = e;

@@ -177,8 +177,9 @@ int fx_emf_upwind_batch(int n, const double *emf_face, const double *emf_cell,
  * block_emf[dir][EDGE_IDX(dir, i, j, k)] too, as the reference stores it.
  * status[m] = 0, or where the reference aborts: 37 edge outside the block's
  * storage, 31 non-finite input, 38 missing face field for eta != 0, 36
- * non-finite curl B or result; 35 a curl read outside the face storage (an
- * out-of-bounds read in the reference).  emf[m] is NaN where status != 0.
+ * non-finite curl B, 39 non-finite result; 35 a curl read outside the face
+ * storage (an out-of-bounds read in the reference).  emf[m] is NaN where
+ * status != 0.
  * Items run concurrently: where one edge appears more than once, its
  * block_emf entry holds the value of one of them, unspecified which. */
 int fx_emf_upwind_block_batch(int n, const int *edge, const double *emf_face,
@@ -348,6 +349,9 @@ int fx_adj_forward(fx_adj *a, const double *W0, double *WT, int nsteps, double d
 int fx_adj_backward(fx_adj *a, const double *gWT, double *gW0, void *stream);
 
 const char *fx_last_error(void);
+/* The reference's abort message for a per-item status code of the batched
+ * entry points (HLLD 11-20, EMF 31 and 35-39); "ok" for 0. */
+const char *fx_status_message(int status);
 /* Library build string (arch, flags). */
 const char *fx_version(void);
 

@@ -161,7 +161,8 @@ def test_emf_block_batch_resistive(gpu):
 def test_emf_block_batch_errors(gpu):
     """Where the reference aborts, a status per edge: edge outside the block
     (37), non-finite input (31), no face arrays for eta != 0 (38), a curl read
-    outside the face storage (35)."""
+    outside the face storage (35), a non-finite result (39: an infinite eta
+    times the curl, src/prj_mhd.c:713-715)."""
     d = dict(np.load(os.path.join(GOLD, "emf_block_vectors.npz")))
     for k in ("edge", "emf_face", "emf_cell", "v_norm", "eta"):
         d[k] = np.ascontiguousarray(d[k][:6])
@@ -176,6 +177,9 @@ def test_emf_block_batch_errors(gpu):
     assert np.isnan(emf[:3]).all() and np.isfinite(emf[3:]).all()
     _, st = _emf_block_call(gpu, d, bf_present=False)
     assert st.tolist()[3:] == [38, 38, 38]
+    d["eta"] = np.full(6, np.inf)
+    emf, st = _emf_block_call(gpu, d)
+    assert st.tolist() == [37, 31, 35, 39, 39, 39] and np.isnan(emf).all()
     d["eta"] = np.zeros(6)
     _, st = _emf_block_call(gpu, d, bf_present=False)
     assert st.tolist() == [37, 31, 0, 0, 0, 0]
