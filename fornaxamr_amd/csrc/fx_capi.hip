@@ -27,7 +27,6 @@ struct fx_mesh {
     int nh;
     double* W[2] = {nullptr, nullptr};
     double* bf[3] = {nullptr, nullptr, nullptr};   // [slot 0, slot 1, spare]
-    double* U1 = nullptr;
     double* fdat = nullptr;
     double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] (unused), [3] fx_calc_dt,
                              // [4] eSSPRK substep dt; from [8] the CFL partials
@@ -228,7 +227,6 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
     a.bf_in = bf_in;
     a.bf0 = m->bf[0];
     a.bf_out = stage == 1 ? m->bf[1] : m->bf[2];
-    a.U1 = m->U1;
     a.fdat = m->fdat;
     a.err = m->err;
     a.dt_ptr = dt_ptr;
@@ -314,7 +312,6 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
             e = hipMalloc(&m->bf[s], 3 * m->cell_bytes);
             if (e == hipSuccess) e = hipMemset(m->bf[s], 0, 3 * m->cell_bytes);
         }
-        if (e == hipSuccess) e = hipMalloc(&m->U1, 6 * m->cell_bytes);
         if (e == hipSuccess) e = hipMalloc(&m->fdat, 9 * m->cell_bytes);
     }
     if (e == hipSuccess && desc->mhd) {
@@ -365,7 +362,6 @@ int fx_mesh_destroy(fx_mesh* m)
     drop_spans(m);
     for (double* p : m->W) (void)hipFree(p);
     for (double* p : m->bf) (void)hipFree(p);
-    (void)hipFree(m->U1);
     (void)hipFree(m->fdat);
     (void)hipFree(m->dev);
     for (int s = 0; s < 2; ++s) {

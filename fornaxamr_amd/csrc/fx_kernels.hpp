@@ -32,7 +32,6 @@ struct MhdStageArgs {
     const double* bf_in;    // face fields of the stage input (3 components)
     const double* bf0;      // face fields at t^n (stage-2 blend)
     double* bf_out;
-    double* U1;             // scratch: hydro part of the updated conserved state (6 vars)
     double* fdat;           // scratch: face EMFs and normal velocities (3 dirs x 3)
     int* err;               // device error word: [code, dir, x, y, z]
     const double* dt_ptr;

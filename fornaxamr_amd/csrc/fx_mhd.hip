@@ -11,14 +11,16 @@
 // Three launches per stage:
 //   mhd_flux_kernel   one workgroup per 16^3 block, marching x-planes like the
 //                     hydro kernel: HLLD on the block's faces, flux divergence,
-//                     RK combination of the hydro conserved variables (U1),
-//                     and the per-face EMF inputs (Bv1, Bv2, normal velocity);
+//                     RK combination of the hydro conserved variables and their
+//                     conversion to rho, v, Ye and the field-free part of the
+//                     internal energy (into the output slot), and the per-face
+//                     EMF inputs (Bv1, Bv2, normal velocity);
 //   mhd_ghostface_kernel  the same face inputs on the one-face transverse layer
 //                     outside the domain that boundary edges read (the
 //                     reference computes it from ghost cells, -1..16 loops);
 //   mhd_ct_kernel     one workgroup per block, marching x-planes: cell EMFs,
 //                     edge EMFs, CT update of the block's faces, cell-centred B
-//                     and cons->prim of the block's cells.
+//                     and the magnetic term of the internal energy.
 // Faces on block boundaries are computed by both neighbours with identical
 // inputs (as the reference does per block); duplicate stores write identical
 // values.
@@ -69,29 +71,6 @@ __device__ __forceinline__ void prim2cons9(const double* w, double* u)
     u[4] = rho * w[EINT] + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3) +
            0.5 * (b1 * b1 + b2 * b2 + b3 * b3);
     u[5] = rho * w[YE];
-}
-
-// cons -> prim with field (src/prj_eos.c:2160-2220)
-__device__ __forceinline__ void cons2prim9(const double* u, const double* b, double* w)
-{
-    double rho = u[0];
-    if (rho == 0.0) {
-#pragma unroll
-        for (int v = 0; v < 9; ++v) w[v] = 0.0;
-        return;
-    }
-    double v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
-    double kinetic = 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
-    double magnetic = 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) / rho;
-    w[0] = rho;
-    w[1] = v1;
-    w[2] = v2;
-    w[3] = v3;
-    w[4] = u[4] / rho - kinetic - magnetic;
-    w[5] = u[5] / rho;
-    w[6] = b[0];
-    w[7] = b[1];
-    w[8] = b[2];
 }
 
 // one HLLD face from 4-cell stencils along the normal; the stencil of
@@ -370,9 +349,28 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
                     u1[v] = 0.5 * u[v] + 0.5 * ut;
                 }
             }
+            // cons -> prim of the parts that do not depend on the new field
+            // (src/prj_eos.c:2160-2220): rho, v, Ye, and u4/rho - kinetic, the
+            // first two terms of the specific internal energy, which the CT
+            // kernel completes by subtracting the magnetic term
+            double w[6];
+            if (u1[0] == 0.0) {
+#pragma unroll
+                for (int v = 0; v < 6; ++v) w[v] = 0.0;
+            } else {
+                const double rho = u1[0];
+                const double v1 = u1[1] / rho, v2 = u1[2] / rho, v3 = u1[3] / rho;
+                const double kinetic = 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
+                w[0] = rho;
+                w[1] = v1;
+                w[2] = v2;
+                w[3] = v3;
+                w[4] = u1[4] / rho - kinetic;
+                w[5] = u1[5] / rho;
+            }
             const int64_t o = own_at(i);
 #pragma unroll
-            for (int v = 0; v < 6; ++v) a.U1[v * vs + o] = u1[v];
+            for (int v = 0; v < 6; ++v) a.Wout[v * vs + o] = w[v];
         }
 #pragma unroll
         for (int v = 0; v < 6; ++v) Fx[v] = Fxn[v];
@@ -702,22 +700,33 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
                         g.dx[0], ex_e[(jj + 1) * EW + kk], ex_e[jj * EW + kk], g.dx[1]);
         }
         __syncthreads();
-        // cell-centred field from the new faces and cons -> prim
+        // cell-centred field from the new faces, and the rest of cons ->
+        // prim (cons2prim9): the flux kernel left rho, v, Ye and
+        // t = u4/rho - kinetic in Wout; eint = t - magnetic, with
+        // magnetic = 0.5 |B|^2 / rho, is the reference's left-to-right
+        // u4/rho - kinetic - magnetic
         {
             double bc[3];
             bc[0] = 0.5 * (bx_lo + bx_hi);
             bc[1] = 0.5 * (nby[j * BS + k] + nby[(j + 1) * BS + k]);
             bc[2] = 0.5 * (nbz[j * EW + k] + nbz[j * EW + k + 1]);
             const int64_t o = cell_off(g, x0 + i, y0 + j, z0 + k);
-            double u[6], w[9];
+            const double rho = a.Wout[o];
+            double eint = 0.0;
+            if (rho != 0.0) {
+                const double b2 = bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2];
+                const double magnetic = 0.5 * b2 / rho;
+                eint = a.Wout[4 * vs + o] - magnetic;
+            } else {
+                bc[0] = bc[1] = bc[2] = 0.0;   // cons2prim9 zeroes every prim of an empty cell
+            }
+            a.Wout[4 * vs + o] = eint;
 #pragma unroll
-            for (int v = 0; v < 6; ++v) u[v] = a.U1[v * vs + o];
-            cons2prim9(u, bc, w);
-#pragma unroll
-            for (int v = 0; v < 9; ++v) a.Wout[v * vs + o] = w[v];
+            for (int c = 0; c < 3; ++c) a.Wout[(6 + c) * vs + o] = bc[c];
             if (STAGE == 2) {
-                // the next step's CFL maximum, from the new state in registers
-                const double d = cfl_denom(g, w[0], w[1], w[2], w[3], w[4]);
+                // the next step's CFL maximum, from the new state
+                const double d = cfl_denom(g, rho, a.Wout[1 * vs + o], a.Wout[2 * vs + o],
+                                           a.Wout[3 * vs + o], eint);
                 if (d > cfl_acc[t]) cfl_acc[t] = d;   // NaN never wins
             }
         }
