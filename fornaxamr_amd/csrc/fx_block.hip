@@ -165,6 +165,16 @@ struct PencilCell {
     double d[2];         // hydro: min one-sided differences of v_t1, v_t2
 };
 
+// The loads of one pencil cell, issued a face ahead of their use: q as in
+// PencilCell, and (hydro) the transverse neighbours of v_t1 and v_t2 that
+// the velocity differences take.
+template <int MHD>
+struct PencilLoads {
+    static constexpr int NHV = MHD ? 9 : 6;
+    double q[NHV + 2];
+    double nb[4];             // v_t1 at -/+ a1, v_t2 at -/+ a2
+};
+
 // grid: (block, direction 0 or 1); hydro 256 pencils (transverse 0..15), MHD
 // 324 (transverse -1..16, for the CT layer) in a 384-thread workgroup.  The
 // pencil threads of a wave read rows along k, coalesced for x- and y-faces;
@@ -206,22 +216,37 @@ __global__ void __launch_bounds__(MHD ? 384 : 256) block_pencil_kernel(
 #pragma unroll
     for (int lv = 0; lv < NHV; ++lv) gvar[lv] = gmap(dir, lv);
 
-    auto load = [&](int n, PencilCell<MHD>& c) {
+    auto issue = [&](int n, PencilLoads<MHD>& r) {
         int i, j, k;
         ijk(n, i, j, k);
         const int o = lidx(i, j, k);
 #pragma unroll
-        for (int lv = 0; lv < NHV; ++lv) c.q[lv] = W[(int64_t)gvar[lv] * vs + o];
-        c.q[NHV] = eos[bidx(3, 0, i, j, k)];
-        c.q[NHV + 1] = eos[bidx(3, 2, i, j, k)];
+        for (int lv = 0; lv < NHV; ++lv) r.q[lv] = W[(int64_t)gvar[lv] * vs + o];
+        r.q[NHV] = eos[bidx(3, 0, i, j, k)];
+        r.q[NHV + 1] = eos[bidx(3, 2, i, j, k)];
+        if (!MHD) {
+            const double* v1 = W + (int64_t)(V1 + a1) * vs;
+            const double* v2 = W + (int64_t)(V1 + a2) * vs;
+            r.nb[0] = v1[o - sa1];
+            r.nb[1] = v1[o + sa1];
+            r.nb[2] = v2[o - sa2];
+            r.nb[3] = v2[o + sa2];
+        }
+    };
+    auto finish = [&](const PencilLoads<MHD>& r, PencilCell<MHD>& c) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) c.q[q] = r.q[q];
         if (!MHD) {
             // velocity deltas of this cell along the two transverse axes
             // (src/prj_flux.c:1967-2036): v_t1 = local 2, v_t2 = local 3
-            const double* v1 = W + (int64_t)(V1 + a1) * vs;
-            const double* v2 = W + (int64_t)(V1 + a2) * vs;
-            c.d[0] = dmin_delta(c.q[2] - v1[o - sa1], v1[o + sa1] - c.q[2]);
-            c.d[1] = dmin_delta(c.q[3] - v2[o - sa2], v2[o + sa2] - c.q[3]);
+            c.d[0] = dmin_delta(c.q[2] - r.nb[0], r.nb[1] - c.q[2]);
+            c.d[1] = dmin_delta(c.q[3] - r.nb[2], r.nb[3] - c.q[3]);
         }
+    };
+    auto load = [&](int n, PencilCell<MHD>& c) {
+        PencilLoads<MHD> r;
+        issue(n, r);
+        finish(r, c);
     };
     // MC half-slopes of the middle cell (the B1 slot of MHD is replaced by bn)
     auto half = [&](const PencilCell<MHD>& m, const PencilCell<MHD>& c, const PencilCell<MHD>& p,
@@ -232,15 +257,19 @@ __global__ void __launch_bounds__(MHD ? 384 : 256) block_pencil_kernel(
 
     PencilCell<MHD> cm, c0, cp;      // cells n-1, n, n+1 at face n
     double hm[NQ], h0[NQ];           // half-slopes of cells n-1 and n
+    PencilLoads<MHD> ahead;          // cell n+1's loads, issued at face n-1
     {
         PencilCell<MHD> cmm;
         load(-2, cmm);
         load(-1, cm);
         load(0, c0);
+        issue(1, ahead);
         half(cmm, cm, c0, hm);
     }
     for (int n = 0; n <= BS; ++n) {
-        load(n + 1, cp);
+        finish(ahead, cp);
+        // cell n+2 (the last pencil cell is 17, the block's outer ghost layer)
+        if (n + 2 <= BS + 1) issue(n + 2, ahead);
         half(cm, c0, cp, h0);
         double WL[NHV], WR[NHV], Fl[NHV], vloc[3];
 #pragma unroll
