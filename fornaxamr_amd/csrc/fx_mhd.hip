@@ -317,6 +317,12 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
 #pragma unroll
             for (int q = 0; q < 9; ++q) hh[q] = a.Win[q * vs + halo_at(i + 2)];
         }
+        // stage 2: W^n of the own cell, for the RK combination below
+        double w0[9];
+        if (STAGE == 2) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) w0[q] = a.W0[q * vs + own_at(i)];
+        }
 
         // ---- divergence (x, y, z order) and the RK combination -------------
         double fd[6];
@@ -338,9 +344,7 @@ __global__ void __launch_bounds__(NT, 2) mhd_flux_kernel(MhdStageArgs a)
 #pragma unroll
                 for (int v = 0; v < 6; ++v) u1[v] = u[v] + dt * (0.0 + fd[v]);
             } else {
-                double w0[9], us[6];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) w0[q] = a.W0[q * vs + own_at(i)];
+                double us[6];
                 prim2cons9(w0, u);
                 prim2cons9(cm, us);
 #pragma unroll
