@@ -28,6 +28,7 @@ struct fx_mesh {
     double* W[2] = {nullptr, nullptr};
     double* bf[3] = {nullptr, nullptr, nullptr};   // [slot 0, slot 1, spare]
     double* fdat = nullptr;
+    double* zring = nullptr;   // MHD: compact copies of the ghost planes z = -1, nz
     double* dev = nullptr;   // [0] dt, [1] dt_prev, [2] (unused), [3] fx_calc_dt,
                              // [4] eSSPRK substep dt; from [8] the CFL partials
     int nblocks = 0;
@@ -218,7 +219,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
     // 0 and 1 and writes the spare buffer, which then becomes slot 0.
     m->cfl_ready = false;
     double* bf_in = stage == 1 ? m->bf[0] : m->bf[1];
-    FX_HIP(launch_boundary_fill(g, Win, bf_in, m->stream));
+    FX_HIP(launch_boundary_fill(g, Win, bf_in, m->zring, m->stream));
     MhdStageArgs a;
     a.g = g;
     a.Win = Win;
@@ -228,6 +229,7 @@ int run_stage(fx_mesh* m, int stage, const double* dt_ptr, double dt)
     a.bf0 = m->bf[0];
     a.bf_out = stage == 1 ? m->bf[1] : m->bf[2];
     a.fdat = m->fdat;
+    a.zring = m->zring;
     a.err = m->err;
     a.dt_ptr = dt_ptr;
     a.dt = dt;
@@ -313,6 +315,7 @@ int fx_mesh_create(const fx_mesh_desc* desc, fx_mesh** out)
             if (e == hipSuccess) e = hipMemset(m->bf[s], 0, 3 * m->cell_bytes);
         }
         if (e == hipSuccess) e = hipMalloc(&m->fdat, 9 * m->cell_bytes);
+        if (e == hipSuccess) e = hipMalloc(&m->zring, zring_doubles(m->g) * sizeof(double));
     }
     if (e == hipSuccess && desc->mhd) {
         int dev_id = 0, ncu = 0;
@@ -363,6 +366,7 @@ int fx_mesh_destroy(fx_mesh* m)
     for (double* p : m->W) (void)hipFree(p);
     for (double* p : m->bf) (void)hipFree(p);
     (void)hipFree(m->fdat);
+    (void)hipFree(m->zring);
     (void)hipFree(m->dev);
     for (int s = 0; s < 2; ++s) {
         (void)hipFree(m->saved[s]);

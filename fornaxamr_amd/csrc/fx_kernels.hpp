@@ -33,6 +33,7 @@ struct MhdStageArgs {
     const double* bf0;      // face fields at t^n (stage-2 blend)
     double* bf_out;
     double* fdat;           // scratch: face EMFs and normal velocities (3 dirs x 3)
+    const double* zring;    // compact copies of the ghost planes z = -1, nz (zring_doubles)
     int* err;               // device error word: [code, dir, x, y, z]
     const double* dt_ptr;
     double dt;
@@ -70,10 +71,26 @@ hipError_t launch_ens_stage(const EnsInst* inst, const EnsBlock* blocks, int nbl
                             int persistent_wgs, int xcd_remap, int xcd_queues, int* queues,
                             hipStream_t s);
 
+// Compact copies of the MHD ghost planes z = -1 and z = nz (the 9 cell
+// variables over x, y in [-2, n + 2)), which the ghost-layer face kernel reads
+// for its x- and y-faces on those planes instead of one double per line of the
+// padded arrays: copy 0 with y fastest, [v][zc][x + 2][y + 2]; copy 1 with x
+// fastest, [v][zc][y + 2][x + 2]; zc = 0 for z = -1, 1 for z = nz.
+__host__ __device__ inline int64_t zring_plane(const Geom& g)
+{
+    return (int64_t)(g.nx + 4) * (g.ny + 4);
+}
+__host__ __device__ inline int64_t zring_doubles(const Geom& g)
+{
+    return 2 * 9 * 2 * zring_plane(g);
+}
+
 // fill the two-cell ghost shell of the domain (periodic / outflow / reflect;
-// MHD: cell B from the clamped outflow face fields).  The MHD stage needs it;
+// MHD: cell B from the clamped outflow face fields), and (zring non-null) the
+// compact copies of its planes z = -1 and z = nz.  The MHD stage needs it;
 // the hydro stage reads out-of-domain cells at their sources.
-hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s);
+hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, double* zring,
+                                hipStream_t s);
 
 hipError_t launch_hydro_stage(const HydroStageArgs& a, int stage, hipStream_t s);
 

@@ -411,8 +411,17 @@ __device__ __forceinline__ bool decode_ghostface(const Geom& g, int64_t id64, in
         const unsigned ring = outer - (unsigned)n[t1] * (unsigned)n[t2];
         const unsigned cnt = (unsigned)(n[d] + 1) * ring;
         if (id < cnt) {
-            const unsigned nd = id / ring;
-            unsigned r = id - nd * ring;
+            // z-faces: the normal index fastest, so that a wave's stencils
+            // (along z) are contiguous; x- and y-faces: the ring position
+            // fastest
+            unsigned nd, r;
+            if (d == 2) {
+                r = id / (unsigned)(n[d] + 1);
+                nd = id - r * (unsigned)(n[d] + 1);
+            } else {
+                nd = id / ring;
+                r = id - nd * ring;
+            }
             int c1, c2;
             // ring enumeration: two full rows (c1 = -1, N) then two side columns
             const unsigned rows = 2u * w2;
@@ -458,9 +467,24 @@ __global__ void __launch_bounds__(256) mhd_ghostface_kernel(MhdStageArgs a, int6
     if (!decode_ghostface(a.g, id, dir, x, y, z)) return;
     const Geom& g = a.g;
     const double gm1 = g.gamma - 1.0;
-    const int S = dir == 0 ? (int)g.sx : (dir == 1 ? (int)g.sy : 1);
+    // the stencil: base (the right cell), the stride along the face normal and
+    // between variables -- in the padded arrays, or, for x- and y-faces on the
+    // ghost planes z = -1 and z = nz, in their compact copies (launch_boundary_fill)
+    int S = dir == 0 ? (int)g.sx : (dir == 1 ? (int)g.sy : 1);
     const double* base = a.Win + cell_off(g, x, y, z);
-    const int64_t vs = g.vstride;
+    int64_t vs = g.vstride;
+    if (dir != 2 && (z == -1 || z == g.nz) && a.zring != nullptr) {
+        const int64_t pl = zring_plane(g);
+        const int zc = z == -1 ? 0 : 1;
+        vs = 2 * pl;
+        if (dir == 0) {
+            S = g.ny + 4;
+            base = a.zring + zc * pl + (int64_t)(x + 2) * (g.ny + 4) + (y + 2);
+        } else {
+            S = g.nx + 4;
+            base = a.zring + 18 * pl + zc * pl + (int64_t)(y + 2) * (g.nx + 4) + (x + 2);
+        }
+    }
     double pq[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {

@@ -30,7 +30,8 @@ __device__ __forceinline__ double bf_clamped(const Geom& g, const double* bf, in
 // (normal velocities negated across reflecting faces) and, for MHD, the cell
 // field from the (ghost) faces.
 __device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
-                                         const double* __restrict__ bf, int x, int y, int z)
+                                         const double* __restrict__ bf,
+                                         double* __restrict__ zring, int x, int y, int z)
 {
     int flip = 0;
     int mx = map_axis(g, 0, g.nx, x, flip);
@@ -39,11 +40,12 @@ __device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
     const int64_t src = cell_off(g, mx, my, mz);
     const int64_t dst = cell_off(g, x, y, z);
     const int64_t vs = g.vstride;
+    double val[9];
 #pragma unroll
     for (int v = 0; v < 6; ++v) {
-        double val = W[v * vs + src];
-        if (v >= 1 && v <= 3 && (flip & (1 << (v - 1)))) val = -val;
-        W[v * vs + dst] = val;
+        val[v] = W[v * vs + src];
+        if (v >= 1 && v <= 3 && (flip & (1 << (v - 1)))) val[v] = -val[v];
+        W[v * vs + dst] = val[v];
     }
     if (g.mhd) {
         // cell B of a ghost cell is the average of its (ghost) faces, as
@@ -52,9 +54,24 @@ __device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
         double b1 = 0.5 * (bf_clamped(g, bf, 0, x, y, z) + bf_clamped(g, bf, 0, x + 1, y, z));
         double b2 = 0.5 * (bf_clamped(g, bf, 1, x, y, z) + bf_clamped(g, bf, 1, x, y + 1, z));
         double b3 = 0.5 * (bf_clamped(g, bf, 2, x, y, z) + bf_clamped(g, bf, 2, x, y, z + 1));
-        W[6 * vs + dst] = b1 / 1.0;
-        W[7 * vs + dst] = b2 / 1.0;
-        W[8 * vs + dst] = b3 / 1.0;
+        val[6] = b1 / 1.0;
+        val[7] = b2 / 1.0;
+        val[8] = b3 / 1.0;
+        W[6 * vs + dst] = val[6];
+        W[7 * vs + dst] = val[7];
+        W[8 * vs + dst] = val[8];
+        if (zring != nullptr && (z == -1 || z == g.nz)) {
+            // the compact copies of the planes z = -1 and z = nz
+            const int64_t pl = zring_plane(g), vstr = 2 * pl;
+            const int zc = z == -1 ? 0 : 1;
+            const int64_t oy = zc * pl + (int64_t)(x + 2) * (g.ny + 4) + (y + 2);
+            const int64_t ox = 18 * pl + zc * pl + (int64_t)(y + 2) * (g.nx + 4) + (x + 2);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) {
+                zring[v * vstr + oy] = val[v];
+                zring[v * vstr + ox] = val[v];
+            }
+        }
     }
 }
 
@@ -71,7 +88,8 @@ __device__ __forceinline__ void fill_one(const Geom& g, double* __restrict__ W,
 // The decomposition uses 32-bit unsigned arithmetic (a shell has < 2^31
 // items): 64-bit division by a run-time divisor is a long software sequence.
 __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restrict__ W,
-                                                const double* __restrict__ bf, int64_t id64)
+                                                const double* __restrict__ bf,
+                                                double* __restrict__ zring, int64_t id64)
 {
     const unsigned nyp = (unsigned)(g.ny + 2 * NG), nzp = (unsigned)(g.nz + 2 * NG);
     const unsigned nx_part = 4u * nyp * nzp;
@@ -82,29 +100,29 @@ __device__ __forceinline__ void fill_shell_cell(const Geom& g, double* __restric
         const int z = (int)(r % nzp) - NG; r /= nzp;
         const int y = (int)(r % nyp) - NG; r /= nyp;
         const int s = (int)r;                     // 0..3
-        fill_one(g, W, bf, s < 2 ? s - NG : g.nx + (s - 2), y, z);
+        fill_one(g, W, bf, zring, s < 2 ? s - NG : g.nx + (s - 2), y, z);
         return;
     }
     if (id < nx_part + ny_part) {
         unsigned r = id - nx_part;
         const int z = (int)(r % nzp) - NG; r /= nzp;
         const int s = (int)(r & 3u); r >>= 2;
-        fill_one(g, W, bf, (int)r, s < 2 ? s - NG : g.ny + (s - 2), z);
+        fill_one(g, W, bf, zring, (int)r, s < 2 ? s - NG : g.ny + (s - 2), z);
         return;
     }
     unsigned r = id - nx_part - ny_part;
     const int c = (int)(r & 3u); r >>= 2;         // 0, 1: z = -2, -1; 2, 3: nz, nz+1
     const int y = (int)(r % (unsigned)g.ny);
     const int x = (int)(r / (unsigned)g.ny);
-    fill_one(g, W, bf, x, y, c < 2 ? c - NG : g.nz + (c - 2));
+    fill_one(g, W, bf, zring, x, y, c < 2 ? c - NG : g.nz + (c - 2));
 }
 
 __global__ void boundary_fill_kernel(Geom g, double* __restrict__ W, const double* __restrict__ bf,
-                                     int64_t total)
+                                     double* __restrict__ zring, int64_t total)
 {
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= total) return;
-    fill_shell_cell(g, W, bf, id);
+    fill_shell_cell(g, W, bf, zring, id);
 }
 
 // CFL denominator of one active cell (src/prj_timeint.c:2163-2182).  The
@@ -604,11 +622,12 @@ hipError_t launch_div_scalar(const double* src, double* dst, double div, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, hipStream_t s)
+hipError_t launch_boundary_fill(const Geom& g, double* W, const double* bf, double* zring,
+                                hipStream_t s)
 {
     const int64_t total = shell_cell_count(g);
     hipLaunchKernelGGL(boundary_fill_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, g, W,
-                       bf, total);
+                       bf, zring, total);
     return hipGetLastError();
 }
 
