@@ -696,6 +696,16 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
         // (the edges of the previous plane, or of the prologue) are behind the
         // barrier that preceded the previous CT step
         cell_fill(s1, i + 1);
+        // stage 1: loads of the last phase for this thread's own cell, issued
+        // here so that their latency overlaps the edge phase -- rho and
+        // u4/rho - kinetic, which the flux kernel left in the output slot
+        // (stage 2 has no VGPRs to spare for them across the edges)
+        double rho_c = 0.0, t_c = 0.0;
+        if (STAGE == 1) {
+            const int64_t oc = cell_off(g, x0 + i, y0 + j, z0 + k);
+            rho_c = a.Wout[oc];
+            t_c = a.Wout[4 * vs + oc];
+        }
         __syncthreads();
         // Each loop has a remainder beyond NT items (33, 16, 16); rotating
         // the thread -> item map by whole waves gives each remainder to a
@@ -739,12 +749,16 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
             bc[1] = 0.5 * (nby[j * BS + k] + nby[(j + 1) * BS + k]);
             bc[2] = 0.5 * (nbz[j * EW + k] + nbz[j * EW + k + 1]);
             const int64_t o = cell_off(g, x0 + i, y0 + j, z0 + k);
-            const double rho = a.Wout[o];
+            if (STAGE == 2) {
+                rho_c = a.Wout[o];
+                t_c = a.Wout[4 * vs + o];
+            }
+            const double rho = rho_c;
             double eint = 0.0;
             if (rho != 0.0) {
                 const double b2 = bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2];
                 const double magnetic = 0.5 * b2 / rho;
-                eint = a.Wout[4 * vs + o] - magnetic;
+                eint = t_c - magnetic;
             } else {
                 bc[0] = bc[1] = bc[2] = 0.0;   // cons2prim9 zeroes every prim of an empty cell
             }
