@@ -186,6 +186,18 @@ def test_async_steps_match_sync(gpu):
     assert np.array_equal(a.get_prim(0), b.get_prim(0))
 
 
+def test_bench_size_256_steps_match_oracle(gpu):
+    """The bench's C2 workload at its full size (256^3 periodic turbulent box,
+    4096 blocks): two RK2 steps, dt and every cell bit-identical to the
+    oracle."""
+    pr = P.turbulent_box(n=(256, 256, 256), seed=1)
+    gm, om = _pair(pr)
+    for s in range(2):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    assert np.array_equal(gm.get_prim(0), om.W0)
+    gm.close()
+
+
 def test_conservation_256(gpu):
     """C2 at full size (256^3, 4096 blocks, 100 RK2 steps): a periodic box
     conserves mass, momentum, energy and Ye to round-off, and the free-stream

@@ -99,6 +99,21 @@ def test_large_mesh_default_schedule(gpu):
     gm.close()
 
 
+def test_bench_size_256_steps_match_oracle(gpu):
+    """The bench's MHD workload at its full size (256^3 magnetized Sedov, 4096
+    blocks; default schedule: CT split 1, ghost-layer faces in order, their
+    stencils on the ghost planes z = -1 / nz from the compact copies): two RK2
+    steps, dt, cells and face fields bit-identical to the oracle."""
+    pr = P.magnetized_sedov(n=(256, 256, 256))
+    gm, om = _pair(pr)
+    for s in range(2):
+        assert gm.step(pr.cfl) == om.step(pr.cfl), s
+    assert ulp_diff(gm.get_prim(0), om.W0) == 0
+    for a, b in zip(gm.get_bf(0), om.bf0):
+        assert np.array_equal(a, b)
+    gm.close()
+
+
 def test_schedule_arguments(gpu):
     from fornaxamr_amd import _native
     pr = P.random_smooth(n=(32, 32, 32), seed=45, bc=(0,) * 6, mhd=True)
