@@ -32,7 +32,8 @@
 #include "fornax_gpu.h"
 
 /* device memory of one call: scalars first, then (MHD) the block's three
- * face arrays for the resistive term */
+ * face arrays for the resistive term; allocated on the first call, held
+ * until the process ends (the calls are serial, as the reference's are) */
 enum { CALL_DOUBLES = 64 };
 static double *g_dev;
 #if PRJ_MHD

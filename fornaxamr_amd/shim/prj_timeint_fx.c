@@ -106,10 +106,16 @@ static void faces_exchange(prj_mesh *m, int to_global)
 }
 #endif
 
+void prj_fx_detach(void);
+
+/* one mesh per process: attaching again releases the previous one first */
 void prj_fx_attach(prj_sim *sim)
 {
     prj_mesh *m = &sim->mesh;
     fx_mesh_desc desc = {{0}};
+    if (g_fx != 0) {
+        prj_fx_detach();
+    }
     const double xl[6] = {m->coord.x1min, m->coord.x1max, m->coord.x2min,
                           m->coord.x2max, m->coord.x3min, m->coord.x3max};
     const int bc[6] = {sim->bc.bc_x1_inner, sim->bc.bc_x1_outer, sim->bc.bc_x2_inner,
