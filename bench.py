@@ -296,12 +296,15 @@ def cpu_baseline(nsteps=6):
                        f"{'serial reference build' if kind == 'reference' else 'C restatement'}")}
 
 
-def run_mesh(pr, bytes_stage, steps, warmup, integrator="rk2"):
+def run_mesh(pr, bytes_stage, steps, warmup, integrator="rk2", schedule=None):
     """Steps of one device-resident mesh; stage kernels event-timed on its stream.
-    Returns zone-updates/s counting every stage (eSSPRK(10,4): 10 substeps)."""
+    Returns zone-updates/s counting every stage (eSSPRK(10,4): 10 substeps).
+    schedule: (ct_split, side_stream) of an MHD mesh, None for the automatic one."""
     from fornaxamr_amd.mesh import GpuMesh
     mesh = GpuMesh.from_problem(pr)
     mesh.set_integrator(integrator)
+    if schedule is not None:
+        mesh.set_mhd_schedule(*schedule)
     stages = 10 if integrator == "essprk10_4" else 2
     for _ in range(warmup):
         mesh.step_async(pr.cfl)
@@ -323,19 +326,23 @@ def run_mesh(pr, bytes_stage, steps, warmup, integrator="rk2"):
     return ncells * stages * steps / elapsed, elapsed / steps * 1000.0, roof
 
 
-def run_mhd_line(n, steps, warmup):
+def run_mhd_line(n, steps, warmup, schedule=None):
     """The MHD stage on the magnetized Sedov problem at n^3 (C3 at 128^3; the
     256^3 line is the headline MHD target): zone-updates/s and the roofline of
-    the stage's kernels (208 / 304 B per zone, event-timed live)."""
+    the stage's kernels (208 / 304 B per zone, event-timed live).  schedule:
+    (ct_split, side_stream) in place of the automatic launch schedule."""
     from fornaxamr_amd import problems as P
     pr = P.magnetized_sedov(n=(n, n, n))
-    value, ms, roof = run_mesh(pr, BYTES_STAGE_MHD, steps, warmup)
+    value, ms, roof = run_mesh(pr, BYTES_STAGE_MHD, steps, warmup, schedule=schedule)
     roof["kernel"] = "mhd_flux_kernel + mhd_ghostface_kernel + mhd_ct_kernel"
     roof["traffic"] = _traffic(f"mhd_{n}")
-    return {"value": value, "unit": "zone-updates/s", "ms_per_step": ms,
+    line = {"value": value, "unit": "zone-updates/s", "ms_per_step": ms,
             "workload": (f"MHD PLM(MC)+HLLD+upwind CT RK2, {n}^3 magnetized Sedov (outflow), "
                          f"{pr.nblocks} blocks of 16^3, CFL 0.4"),
             "roofline": roof}
+    if schedule is not None:
+        line["schedule"] = {"ct_split": schedule[0], "side_stream": schedule[1]}
+    return line
 
 
 def _traffic(name):
@@ -359,6 +366,10 @@ def main():
                          "default eSSPRK(10,4); stage kernels are its forward-Euler substeps)")
     ap.add_argument("--n", type=int, default=None, help="cells per axis (multiple of 16); "
                     "default 256 (hydro), 128 (mhd)")
+    ap.add_argument("--mhd-schedule", default=None, metavar="CT,SIDE",
+                    help="MHD launch schedule instead of the automatic one: CT workgroups "
+                         "per block (1, 2, 4) and the ghost-layer faces in order (0) or on "
+                         "a second stream (1), e.g. 1,0 (fx_mesh_set_mhd_schedule)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble figure")
     ap.add_argument("--no-mhd", action="store_true",
@@ -402,8 +413,10 @@ def main():
         print(json.dumps({"ensemble": ens}), flush=True)
         return
 
+    schedule = (tuple(int(v) for v in args.mhd_schedule.split(","))
+                if args.mhd_schedule else None)
     if mhd:
-        line = run_mhd_line(n, args.steps, args.warmup)
+        line = run_mhd_line(n, args.steps, args.warmup, schedule)
         value, ms, roof, workload = (line["value"], line["ms_per_step"], line["roofline"],
                                      line["workload"])
         pr = P.magnetized_sedov(n=(n, n, n))
@@ -437,7 +450,7 @@ def main():
         out["mhd"] = {}
         for m in (256, 128):
             _progress(f"mhd {m}^3")
-            out["mhd"][str(m)] = run_mhd_line(m, args.steps, args.warmup)
+            out["mhd"][str(m)] = run_mhd_line(m, args.steps, args.warmup, schedule)
     if not mhd and not args.no_ensemble:
         _progress("ensemble")
         out["batched"] = run_ensemble(args.steps, args.warmup)
