@@ -653,11 +653,12 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
         return edge(0, gx, gy, gz, ef, ec, vn, mk);
     };
     // CT value of one face (src/prj_timeint.c:415-495); e_hi/e_lo pairs in
-    // the reference's order, `dn` the two cell widths
+    // the reference's order, `dn` the two cell widths; the face's stage-input
+    // field is `pre` where `use_pre`, else loaded here
     auto ct = [&](int dir, int gx, int gy, int gz, double ea1, double ea0, double da, double eb1,
-                  double eb0, double db) -> double {
+                  double eb0, double db, bool use_pre = false, double pre = 0.0) -> double {
         const int64_t o = dir * g.fvstride + cell_off(g, gx, gy, gz);
-        double value = a.bf_in[o];
+        double value = use_pre ? pre : a.bf_in[o];
         value += -dt * (ea1 - ea0) / da;
         value += dt * (eb1 - eb0) / db;
         if (!isfinite(value)) set_error(a.err, 34, dir, gx, gy, gz, STAGE);
@@ -722,20 +723,34 @@ __global__ void __launch_bounds__(NT, 4) mhd_ct_kernel(MhdStageArgs a)
             int jj = e / BS, kk = e % BS;
             ez_e[e1][e] = edge_z(i + 1, jj, kk, s0, s1);
         }
+        // stage 2: the stage-input fields of this thread's first face in each
+        // CT loop below (Bx(i+1) of its own cell, By / Bz of plane i), loaded
+        // before the barrier so that their latency overlaps the wait (stage 1,
+        // which holds rho and t of its cell here, has no VGPRs for them)
+        const int fy0 = (t + 64) & (NT - 1), fz0 = (t + 128) & (NT - 1);
+        double bx_in = 0.0, by_in = 0.0, bz_in = 0.0;
+        if (STAGE == 2) {
+            const int64_t fv = g.fvstride;
+            bx_in = a.bf_in[cell_off(g, x0 + i + 1, y0 + j, z0 + k)];
+            by_in = a.bf_in[fv + cell_off(g, x0 + i, y0 + fy0 / BS, z0 + fy0 % BS)];
+            bz_in = a.bf_in[2 * fv + cell_off(g, x0 + i, y0 + fz0 / EW, z0 + fz0 % EW)];
+        }
         __syncthreads();
         // CT of Bx(i+1), By(plane i), Bz(plane i)
         double bx_hi = ct(0, x0 + i + 1, y0 + j, z0 + k, ez_e[e1][(j + 1) * BS + k],
                           ez_e[e1][j * BS + k], g.dx[1], ey_e[e1][j * EW + k + 1],
-                          ey_e[e1][j * EW + k], g.dx[2]);
-        for (int f = (t + 64) & (NT - 1); f < EW * BS; f += NT) {   // remainder: wave 3
+                          ey_e[e1][j * EW + k], g.dx[2], STAGE == 2, bx_in);
+        for (int f = fy0; f < EW * BS; f += NT) {   // remainder: wave 3
             int jj = f / BS, kk = f % BS;   // By(i, jj 0..16, kk 0..15)
             nby[f] = ct(1, x0 + i, y0 + jj, z0 + kk, ex_e[jj * EW + kk + 1], ex_e[jj * EW + kk],
-                        g.dx[2], ez_e[e1][jj * BS + kk], ez_e[e0][jj * BS + kk], g.dx[0]);
+                        g.dx[2], ez_e[e1][jj * BS + kk], ez_e[e0][jj * BS + kk], g.dx[0],
+                        STAGE == 2 && f == fy0, by_in);
         }
-        for (int f = (t + 128) & (NT - 1); f < BS * EW; f += NT) {  // remainder: wave 2
+        for (int f = fz0; f < BS * EW; f += NT) {  // remainder: wave 2
             int jj = f / EW, kk = f % EW;   // Bz(i, jj 0..15, kk 0..16)
             nbz[f] = ct(2, x0 + i, y0 + jj, z0 + kk, ey_e[e1][jj * EW + kk], ey_e[e0][jj * EW + kk],
-                        g.dx[0], ex_e[(jj + 1) * EW + kk], ex_e[jj * EW + kk], g.dx[1]);
+                        g.dx[0], ex_e[(jj + 1) * EW + kk], ex_e[jj * EW + kk], g.dx[1],
+                        STAGE == 2 && f == fz0, bz_in);
         }
         __syncthreads();
         // cell-centred field from the new faces, and the rest of cons ->
