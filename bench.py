@@ -236,7 +236,8 @@ def run_per_block(nblocks=4096, reps=10):
     ms = a.elapsed_time(b) / reps
     gbs = BYTES_BLOCK * nblocks / (ms * 1e-3) / 1e9
     host = {name: _host_block_call_ms(lib, mhd) for name, mhd in (("hydro", 0), ("mhd", 1))}
-    return {"kernel": "block_flux_kernel<0> (prj_flux_update, reference block layouts)",
+    return {"kernel": "block_pencil_kernel<0> (x, y faces) + block_face_kernel<0, 2> (z faces): "
+                      "prj_flux_update on the reference's block layouts",
             "blocks": nblocks, "ms_per_launch": ms,
             "host_array_call_ms": host,
             "faces_per_s": BLOCK_FACES * nblocks / (ms * 1e-3),
