@@ -175,6 +175,38 @@ struct PencilLoads {
     double nb[4];             // v_t1 at -/+ a1, v_t2 at -/+ a2
 };
 
+// The hydro face outputs of one wave (4 rows of 16 pencils) leave through a
+// wave-private LDS tile: written [row][var][k] by the computing lanes, read
+// back as 16-byte pairs (k, k+1) of one variable in the arrays' own order
+// (AoSoA group, variable, slot pair), so that a wave stores each row's
+// 16 x 6 fluxes as 48 16-byte pieces in increasing address order (3 store
+// instructions per face for the fluxes, 2 for the face velocities) where each
+// variable's per-lane store scattered over ~20 cache lines.  A row's first and
+// last AoSoA group hold two active k (slots 2-3 and 0-1); the ghost slots are
+// not written.  Row pitches 112 / 48 doubles put the two rows of a half-wave
+// in complementary LDS banks.
+constexpr int SFP = 112;   // flux tile row pitch (6 x 16 + 16)
+constexpr int SVP = 48;    // face-velocity tile row pitch (3 x 16)
+
+// Piece w (0 .. 8 nc - 1, in address order) of one row of an AoSoA array of
+// nc variables over the 16 active k: its variable and the first k of its pair.
+__device__ __forceinline__ void row_piece(int nc, int w, int& v, int& k0)
+{
+    if (w < nc) {                    // group 0: slots 2-3 = k 0, 1
+        v = w;
+        k0 = 0;
+    } else if (w < 7 * nc) {         // groups 1..3: slot pairs 0-1 and 2-3
+        const int t = w - nc;
+        const int g = 1 + t / (2 * nc);
+        const int rem = t % (2 * nc);
+        v = rem >> 1;
+        k0 = 4 * g - 2 + 2 * (rem & 1);
+    } else {                         // group 4: slots 0-1 = k 14, 15
+        v = w - 7 * nc;
+        k0 = 14;
+    }
+}
+
 // grid: (block, direction 0 or 1); hydro 256 pencils (transverse 0..15), MHD
 // 324 (transverse -1..16, for the CT layer) in a 384-thread workgroup.  The
 // pencil threads of a wave read rows along k, coalesced for x- and y-faces;
@@ -200,6 +232,11 @@ __global__ void __launch_bounds__(MHD ? 384 : 256) block_pencil_kernel(
     double* F = bp->flux[dir];
     double* vr = bp->v_riemann[dir];
     const double* bfn = MHD ? bp->bf[dir] : nullptr;
+    // hydro: the wave's staging tiles; the paired stores need 16-byte
+    // aligned output arrays (else each lane stores its own face)
+    __shared__ __align__(16) double stg_f[MHD ? 1 : NT / 64][MHD ? 1 : 4 * SFP];
+    __shared__ __align__(16) double stg_v[MHD ? 1 : NT / 64][MHD ? 1 : 4 * SVP];
+    const bool paired = !MHD && ((reinterpret_cast<uintptr_t>(F) | reinterpret_cast<uintptr_t>(vr)) & 15) == 0;
     double* bv1o = MHD ? bp->Bv1[dir] : nullptr;
     double* bv2o = MHD ? bp->Bv2[dir] : nullptr;
     // transverse coordinates: the fastest-varying one is the array's fastest
@@ -306,7 +343,49 @@ __global__ void __launch_bounds__(MHD ? 384 : 256) block_pencil_kernel(
             hllc(WL, WR, pL, pR, gL, gR, du, dmin_delta(cm.d[0], c0.d[0]),
                  dmin_delta(cm.d[1], c0.d[1]), Fl, vloc);
         }
-        if (ok) {
+        if (!MHD && paired) {
+            const int wv = pid >> 6, lane = pid & 63, row = (pid >> 4) & 3, kk = pid & 15;
+            double* sf = stg_f[MHD ? 0 : wv];
+            double* sv = stg_v[MHD ? 0 : wv];
+#pragma unroll
+            for (int lv = 0; lv < NHV; ++lv) sf[row * SFP + gvar[lv] * 16 + kk] = Fl[lv];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) sv[row * SVP + (gvar[1 + r] - 1) * 16 + kk] = vloc[r];
+            // the tile is read by other lanes of this wave only: LDS operations
+            // of a wave complete in order, the fences keep the compiler's order
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // 4 rows x 48 flux pieces, 4 rows x 24 velocity pieces
+#pragma unroll 1
+            for (int sidx = 0; sidx < 3; ++sidx) {
+                const int c = sidx * 64 + lane;
+                const int rr = c / (8 * NHV), w = c % (8 * NHV);
+                int v, k0;
+                row_piece(NHV, w, v, k0);
+                const int prow = (pid >> 6) * 4 + rr;        // pf of row rr
+                const int pi = dir == 0 ? n : prow, pj = dir == 0 ? prow : n;
+                const double2 val = *reinterpret_cast<const double2*>(&sf[rr * SFP + v * 16 + k0]);
+                *reinterpret_cast<double2*>(&F[bidx(NHV, v, pi, pj, k0)]) = val;
+            }
+#pragma unroll 1
+            for (int sidx = 0; sidx < 2; ++sidx) {
+                const int c = sidx * 64 + lane;
+                if (c < 4 * 24) {
+                    const int rr = c / 24, w = c % 24;
+                    int v, k0;
+                    row_piece(3, w, v, k0);
+                    const int prow = (pid >> 6) * 4 + rr;
+                    const int pi = dir == 0 ? n : prow, pj = dir == 0 ? prow : n;
+                    const double2 val =
+                        *reinterpret_cast<const double2*>(&sv[rr * SVP + v * 16 + k0]);
+                    *reinterpret_cast<double2*>(&vr[bidx(3, v, pi, pj, k0)]) = val;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else if (ok) {
 #pragma unroll
             for (int lv = 0; lv < NHV; ++lv) F[bidx(NHV, gvar[lv], i, j, k)] = Fl[lv];
 #pragma unroll

@@ -83,7 +83,11 @@ def test_flux_update_block_matches_reference(gpu, case):
     assert _sha(fd) == g["flux_div_sha"]
 
 
-def test_batched_blocks_equal_single_block(gpu):
+@pytest.mark.parametrize("shift", [0, 1], ids=["aligned", "offset8"])
+def test_batched_blocks_equal_single_block(gpu, shift):
+    """Several blocks in one launch give the single-block bits; output arrays
+    one double off a 16-byte boundary (shift 1) take the kernel's per-lane
+    store path instead of its paired 16-byte stores."""
     import torch
     from fornaxamr_amd import _native
     cases = [(0, 11, 0), (0, 12, 1), (0, 13, 1)]
@@ -95,10 +99,11 @@ def test_batched_blocks_equal_single_block(gpu):
         singles.append(_run_block(gpu, mhd, slot, slab, eos, bf))
         W = torch.from_numpy(np.ascontiguousarray(slab.reshape(6, -1))).to(dev)
         E = torch.from_numpy(eos).to(dev)
-        F = [torch.full((6 * O.RB_CELLS,), float("nan"), dtype=torch.float64, device=dev)
-             for _ in range(3)]
-        V = [torch.full((3 * O.RB_CELLS,), float("nan"), dtype=torch.float64, device=dev)
-             for _ in range(3)]
+        F = [torch.full((6 * O.RB_CELLS + shift,), float("nan"), dtype=torch.float64,
+                        device=dev)[shift:] for _ in range(3)]
+        V = [torch.full((3 * O.RB_CELLS + shift,), float("nan"), dtype=torch.float64,
+                        device=dev)[shift:] for _ in range(3)]
+        assert all((t.data_ptr() % 16 == 0) == (shift == 0) for t in F + V)
         io = _native.BlockIO()
         io.W = W.data_ptr() + slot * O.RB_CELLS * 8
         io.var_stride = 2 * O.RB_CELLS
