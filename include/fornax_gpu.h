@@ -196,7 +196,10 @@ int fx_emf_upwind_block_batch(int n, const int *edge, const double *emf_face,
  *   flux[3]    out, VIDX AoSoA, 6 (hydro) or 9 (MHD) conserved variables
  *   v_riemann  out, VRIDX AoSoA; Bv1/Bv2 out, IDX (MHD)
  * Faces written are the reference's (src/prj_flux.c:2221-2262): normal index
- * 0..16; transverse 0..15 (hydro) or -1..16 (MHD); nothing else is touched. */
+ * 0..16; transverse 0..15 (hydro) or -1..16 (MHD); nothing else is touched.
+ * Hydro flux and v_riemann arrays on 16-byte boundaries (as hipMalloc and
+ * malloc return them) are written with paired 16-byte stores; others with one
+ * store per value, same results. */
 typedef struct fx_block_io {
     const double *W;
     long long var_stride;
