@@ -219,6 +219,8 @@ __global__ void __launch_bounds__(MHD ? 384 : 256) block_pencil_kernel(
     constexpr int NQ = NHV + 2;
     constexpr int T = MHD ? 18 : 16;      // pencils per transverse axis
     constexpr int LO = MHD ? -1 : 0;
+    // the hydro output tiles take a wave as 4 whole rows of 16 pencils
+    static_assert(MHD || (T * T == NT && T == 16), "hydro pencil layout");
     const int pid = threadIdx.x;
     if (pid >= T * T) return;
     const fx_block_io* bp = blocks + blockIdx.x;
